@@ -1,0 +1,203 @@
+/*
+ * pypo_hip.h -- C ABI of libpypohip.so, the MI355X (gfx950) physical-optics
+ * and ray-trace library.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in symbols.  Same names, argument types and argument meaning as
+ *     the reference's FP64 CPU library (src/CPU/InterfaceCPU.h:24-64).  All
+ *     buffers are host memory owned by the caller (ctypes arrays); the
+ *     library generates the source and target grids itself, exactly like
+ *     src/CPU/InterfaceCPU.cpp:29-243, and writes the result in place.
+ *     `numThreads` is accepted and ignored (the GPU launch shape is chosen
+ *     by the library).  Calls are synchronous and never exit the process:
+ *     failures are reported through pypo_hip_last_error().
+ *
+ *  2. pypo_hip_* extensions.  Device-resident variants that take device
+ *     pointers and a hipStream_t (passed as void*), for chaining JM -> FF
+ *     without host round trips, batched sweeps and the autograd wrapper.
+ *
+ * Struct layouts are byte-identical to src/include/Structs.h:29-161 so that
+ * the reference's ctypes mirrors (src/PyPO/Structs.py:9-141) bind unchanged.
+ * Lengths are in mm, k in 1/mm, t_direction = -1 forward / +1 backward
+ * (src/PyPO/BindCPU.py:117-121).
+ */
+#ifndef PYPO_HIP_H
+#define PYPO_HIP_H
+
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- structs: layout of src/include/Structs.h:29-161 ------------------ */
+
+/* One complex scalar field, SoA real/imag.  Structs.h:29-32 */
+typedef struct arrC1 { double *x, *y; } arrC1;
+
+/* Two complex 3-vectors per point (J,M or E,H), SoA.  Structs.h:42-57 */
+typedef struct c2Bundle {
+    double *r1x, *r1y, *r1z, *i1x, *i1y, *i1z;
+    double *r2x, *r2y, *r2z, *i2x, *i2y, *i2z;
+} c2Bundle;
+
+/* Four complex 3-vectors (J,M,E,H).  Structs.h:59-88 */
+typedef struct c4Bundle {
+    double *r1x, *r1y, *r1z, *i1x, *i1y, *i1z;
+    double *r2x, *r2y, *r2z, *i2x, *i2y, *i2z;
+    double *r3x, *r3y, *r3z, *i3x, *i3y, *i3z;
+    double *r4x, *r4y, *r4z, *i4x, *i4y, *i4z;
+} c4Bundle;
+
+/* Two complex 3-vectors and one real 3-vector (E_r, H_r, S_r).  Structs.h:90-108 */
+typedef struct c2rBundle {
+    double *r1x, *r1y, *r1z, *i1x, *i1y, *i1z;
+    double *r2x, *r2y, *r2z, *i2x, *i2y, *i2z;
+    double *r3x, *r3y, *r3z;
+} c2rBundle;
+
+/* Surface parameters, passed BY VALUE.  Structs.h:111-132
+ * type: 0 paraboloid, 1 hyperboloid, 2 ellipsoid, 3 plane.
+ * gmode: 0 xy, 1 uv, 2 AoE.  transf: 4x4 row-major affine matrix. */
+typedef struct reflparams {
+    double *coeffs;       /* a, b, c */
+    double *lxu;          /* lower/upper x or u (or Az) */
+    double *lyv;          /* lower/upper y or v (or El) */
+    int *n_cells;         /* grid size along x/u and y/v */
+    bool flip;
+    int gmode;
+    double *gcenter;
+    double ecc_uv;
+    double rot_uv;        /* degrees */
+    int type;
+    double *transf;
+    double rms;
+    unsigned int rms_seed;
+} reflparams;
+
+/* Realised grid: points, unit normals and area elements.  Structs.h:135-148 */
+typedef struct reflcontainer {
+    int size;
+    double *x, *y, *z, *nx, *ny, *nz, *area;
+} reflcontainer;
+
+/* Ray frame: positions and directions.  Structs.h:151-161 */
+typedef struct cframe {
+    int size;
+    double *x, *y, *z, *dx, *dy, *dz;
+} cframe;
+
+/* ---- 1. drop-in FP64 entry points (src/CPU/InterfaceCPU.h:24-64) ------ */
+
+/* PEC currents J = 2 n x H_inc, M = 0 on the target.  InterfaceCPU.h:24-28,
+ * InterfaceCPU.cpp:29-42 */
+void propagateToGrid_JM(c2Bundle *res, reflparams source, reflparams target,
+                        reflcontainer *cs, reflcontainer *ct, c2Bundle *currents,
+                        double k, int numThreads, double epsilon, double t_direction);
+
+/* Incident E, H on the target.  InterfaceCPU.h:30-34, InterfaceCPU.cpp:64-77 */
+void propagateToGrid_EH(c2Bundle *res, reflparams source, reflparams target,
+                        reflcontainer *cs, reflcontainer *ct, c2Bundle *currents,
+                        double k, int numThreads, double epsilon, double t_direction);
+
+/* PEC J, M plus incident E, H (res->1,2 = J,M; res->3,4 = E,H).
+ * InterfaceCPU.h:36-40, InterfaceCPU.cpp:100-113 */
+void propagateToGrid_JMEH(c4Bundle *res, reflparams source, reflparams target,
+                          reflcontainer *cs, reflcontainer *ct, c2Bundle *currents,
+                          double k, int numThreads, double epsilon, double t_direction);
+
+/* Reflected E_r, H_r and reflected Poynting direction S_r.
+ * InterfaceCPU.h:42-46, InterfaceCPU.cpp:136-149 */
+void propagateToGrid_EHP(c2rBundle *res, reflparams source, reflparams target,
+                         reflcontainer *cs, reflcontainer *ct, c2Bundle *currents,
+                         double k, int numThreads, double epsilon, double t_direction);
+
+/* Scalar field.  InterfaceCPU.h:48-52, InterfaceCPU.cpp:171-184 */
+void propagateToGrid_scalar(arrC1 *res, reflparams source, reflparams target,
+                            reflcontainer *cs, reflcontainer *ct, arrC1 *field,
+                            double k, int numThreads, double epsilon, double t_direction);
+
+/* Far field E, H on an (polar, azimuth) target grid.  InterfaceCPU.h:54-58,
+ * InterfaceCPU.cpp:206-219 */
+void propagateToFarField(c2Bundle *res, reflparams source, reflparams target,
+                         reflcontainer *cs, reflcontainer *ct, c2Bundle *currents,
+                         double k, int numThreads, double epsilon, double t_direction);
+
+/* Ray trace fr_in onto the surface ctp; epsilon is the Newton tolerance and
+ * t0 the starting parameter.  InterfaceCPU.h:60-61, InterfaceCPU.cpp:236-243.
+ * Like the reference, fr_in is transformed into the surface frame and back
+ * (RayTrace.h:284-304), so it is left numerically as it was. */
+void propagateRays(reflparams ctp, cframe *fr_in, cframe *fr_out,
+                   int numThreads, double epsilon, double t0);
+
+/* Surface grid generation (device kernels, FP64) into host arrays, same
+ * contract as src/common/InterfaceReflector.h:21-22 (generateGrid).
+ * transform applies refl.transf; spheric maps AoE (Az, El) to
+ * (polar, azimuth) angles in radians.  The PO entry points above call it
+ * with transform = spheric = true. */
+void generateGrid(reflparams refl, reflcontainer *container, bool transform, bool spheric);
+
+/* ---- 2. extensions --------------------------------------------------- */
+
+enum pypo_hip_mode {
+    PYPO_MODE_JM = 0, PYPO_MODE_EH = 1, PYPO_MODE_JMEH = 2,
+    PYPO_MODE_EHP = 3, PYPO_MODE_FF = 4, PYPO_MODE_SCALAR = 5
+};
+
+enum pypo_hip_precision {
+    PYPO_PREC_MIXED = 0,   /* FP32 integrand, FP64 distance and phase */
+    PYPO_PREC_FP64 = 1     /* everything in FP64 */
+};
+
+/* Last error of the calling thread: returns 0 if none (and clears it),
+ * otherwise a non-zero code and the message copied into buf (n bytes). */
+int pypo_hip_last_error(char *buf, int n);
+
+/* Library version string. */
+const char *pypo_hip_version(void);
+
+/* Number of visible HIP devices (0 when none).  Does not create a context. */
+int pypo_hip_device_count(void);
+
+/* Precision used by the drop-in PO entry points (default PYPO_PREC_MIXED).
+ * Returns the previous value. */
+int pypo_hip_set_precision(int precision);
+
+/* Maximum Newton iterations per ray (default 50; the reference loops until
+ * converged).  Returns the previous value. */
+int pypo_hip_set_rt_max_iter(int max_iter);
+
+/* Device-resident PO propagation.  All pointers are device pointers; the
+ * structs themselves live in host memory.  Grids must already be generated
+ * (pypo_hip_generate_grid_dev).  Source currents: c2Bundle (or arrC1 for
+ * the scalar mode).  res is c2Bundle (JM, EH, FF), c4Bundle (JMEH),
+ * c2rBundle (EHP) or arrC1 (scalar) according to mode.  gmode/ncx/ncy are
+ * the source grid's gmode and n_cells (trapezoid weights,
+ * Propagation.h:1123-1135).  stream may be NULL (library stream).
+ * Asynchronous on `stream`.  Returns 0 on success. */
+int pypo_hip_propagate_dev(int mode, void *res, const reflcontainer *cs,
+                           const reflcontainer *ct, const void *currents,
+                           int gmode, int ncx, int ncy, double k, double epsilon,
+                           double t_direction, int precision, void *stream);
+
+/* Device-resident grid generation into device arrays of container. */
+int pypo_hip_generate_grid_dev(reflparams refl, reflcontainer *container, void *stream);
+
+/* Device-resident ray trace through n_surf surfaces in sequence: frames[0]
+ * is the input (device arrays, left unchanged), frames[i+1] receives the
+ * result on surface i.  tol/t0 as in propagateRays.  Returns 0 on success. */
+int pypo_hip_trace_dev(const reflparams *surfs, int n_surf, cframe *frames,
+                       double tol, double t0, void *stream);
+
+/* Mean kernel time (ms) of the last pypo_hip_propagate_dev / trace call's
+ * dominant kernel, measured with HIP events on the launch stream, when
+ * timing is enabled by pypo_hip_set_timing(1).  Returns -1 if unavailable. */
+int pypo_hip_set_timing(int enable);
+double pypo_hip_last_kernel_ms(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYPO_HIP_H */

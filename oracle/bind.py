@@ -1,0 +1,158 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes driver for the CPU checkers.
+
+  kind="port" -> oracle/liboracle.so, this project's FP64 restatement of the
+                 reference CPU path (pypo_oracle.cpp), symbols "oracle_*".
+  kind="ref"  -> oracle/_ref/libpypocpu_ref.so, the reference's own C++
+                 sources (src/CPU/InterfaceCPU.cpp & co.) compiled by
+                 oracle/Makefile; present only where it was built.
+
+Used only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline.
+The call sequence mirrors src/PyPO/BindCPU.py:82-254 (PyPO_CPUd, RT_CPUd).
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+from pypo_amd import marshal as MU
+from pypo_amd import structs as PS
+from pypo_amd.geometry import gridsize
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PATHS = {
+    "port": os.path.join(_HERE, "liboracle.so"),
+    "ref": os.path.join(_HERE, "_ref", "libpypocpu_ref.so"),
+}
+_PREFIX = {"port": "oracle_", "ref": ""}
+_cache = {}
+
+_MODE_SYMBOL = {
+    "JM": ("propagateToGrid_JM", "c2Bundle"),
+    "EH": ("propagateToGrid_EH", "c2Bundle"),
+    "JMEH": ("propagateToGrid_JMEH", "c4Bundle"),
+    "EHP": ("propagateToGrid_EHP", "c2rBundle"),
+    "FF": ("propagateToFarField", "c2Bundle"),
+    "scalar": ("propagateToGrid_scalar", "arrC1"),
+}
+
+
+def available(kind):
+    return os.path.exists(PATHS[kind])
+
+
+class Lib:
+    def __init__(self, kind):
+        if not available(kind):
+            raise OSError(f"{PATHS[kind]} is not built (make -C oracle{' ref' if kind == 'ref' else ''})")
+        self.kind = kind
+        self.pre = _PREFIX[kind]
+        self.cdll = ctypes.CDLL(PATHS[kind])
+        P = ctypes.POINTER
+        for sym, bundle in _MODE_SYMBOL.values():
+            f = getattr(self.cdll, self.pre + sym)
+            src_t = PS.arrC1 if bundle == "arrC1" else PS.c2Bundle
+            f.argtypes = [P(getattr(PS, bundle)), PS.reflparams, PS.reflparams, P(PS.reflcontainer),
+                          P(PS.reflcontainer), P(src_t), ctypes.c_double, ctypes.c_int,
+                          ctypes.c_double, ctypes.c_double]
+            f.restype = None
+        rays = getattr(self.cdll, self.pre + "propagateRays")
+        rays.argtypes = [PS.reflparams, P(PS.cframe), P(PS.cframe), ctypes.c_int, ctypes.c_double,
+                         ctypes.c_double] + ([ctypes.c_int] if kind == "port" else [])
+        rays.restype = None
+        gg = getattr(self.cdll, self.pre + "generateGrid")
+        gg.argtypes = [PS.reflparams, P(PS.reflcontainer), ctypes.c_bool, ctypes.c_bool]
+        gg.restype = None
+        if kind == "port":
+            fs = self.cdll.oracle_farfield_subset
+            fs.argtypes = [P(PS.c2Bundle), PS.reflparams, P(PS.reflcontainer), P(PS.reflcontainer),
+                           P(PS.c2Bundle), P(ctypes.c_int), ctypes.c_int, ctypes.c_double,
+                           ctypes.c_int, ctypes.c_double, ctypes.c_double]
+            fs.restype = None
+
+    def fn(self, name):
+        return getattr(self.cdll, self.pre + name)
+
+
+def load(kind="port"):
+    if kind not in _cache:
+        _cache[kind] = Lib(kind)
+    return _cache[kind]
+
+
+def default_threads():
+    return max(1, os.cpu_count() or 1)
+
+
+def generate_grid(lib, refl, transform=True, spheric=True):
+    nx, ny = gridsize(refl)
+    c = MU.alloc_container(nx * ny)
+    lib.fn("generateGrid")(MU.fill_reflparams(refl), ctypes.byref(c), transform, spheric)
+    return MU.container_to_grids(c, (nx, ny))
+
+
+def propagate(lib, mode, source, target, src_obj, k, nthreads=None, epsilon=1.0, exp="fwd"):
+    """One PO propagation through the checker library (BindCPU.PyPO_CPUd).
+    Returns (result, source_grids, target_grids); result types as PyPO_CPUd."""
+    nthreads = nthreads or default_threads()
+    sym, bundle = _MODE_SYMBOL[mode]
+    gs = int(np.prod(gridsize(source)))
+    gt = int(np.prod(gridsize(target)))
+    shape = gridsize(target)
+    csp, ctp = MU.fill_reflparams(source), MU.fill_reflparams(target)
+    cs, ct = MU.alloc_container(gs), MU.alloc_container(gt)
+    tdir = -1.0 if exp == "fwd" else 1.0
+    src = MU.fill_arrc1(src_obj, gs) if mode == "scalar" else MU.fill_c2bundle(src_obj, gs)
+    res = MU.alloc_bundle(bundle, gt)
+    lib.fn(sym)(ctypes.byref(res), csp, ctp, ctypes.byref(cs), ctypes.byref(ct), ctypes.byref(src),
+                float(k), int(nthreads), float(epsilon), tdir)
+    if mode == "JM":
+        out = MU.c2bundle_to_obj(res, shape, "currents")
+    elif mode in ("EH", "FF"):
+        out = MU.c2bundle_to_obj(res, shape, "fields")
+    elif mode == "JMEH":
+        out = list(MU.c4bundle_to_objs(res, shape))
+    elif mode == "EHP":
+        out = list(MU.c2rbundle_to_objs(res, shape))
+    else:
+        out = MU.arrc1_to_obj(res, shape)
+    return out, MU.container_to_grids(cs, gridsize(source)), MU.container_to_grids(ct, shape)
+
+
+def trace(lib, target, fr, nthreads=None, tol=1e-3, t0=1.0, max_iter=0):
+    """Ray trace through the checker library (BindCPU.RT_CPUd).  max_iter is
+    only honoured by the port (0 = reference behaviour, no cap)."""
+    nthreads = nthreads or default_threads()
+    inp = MU.fill_cframe(fr)
+    out = MU.alloc_cframe(fr.size)
+    args = [MU.fill_reflparams(target), ctypes.byref(inp), ctypes.byref(out), int(nthreads),
+            float(tol), float(t0)]
+    if lib.kind == "port":
+        args.append(int(max_iter))
+    lib.fn("propagateRays")(*args)
+    return MU.cframe_to_frame(out)
+
+
+def farfield_subset(lib, source, target, currents, k, idx, nthreads=None, epsilon=1.0, exp="fwd"):
+    """FF on targets idx only (port only).  Grids are generated here, outside
+    the timed call.  Returns (6, len(idx)) complex array and the seconds the
+    native call took."""
+    import time
+    nthreads = nthreads or default_threads()
+    gs = int(np.prod(gridsize(source)))
+    gt = int(np.prod(gridsize(target)))
+    csp, ctp = MU.fill_reflparams(source), MU.fill_reflparams(target)
+    cs, ct = MU.alloc_container(gs), MU.alloc_container(gt)
+    lib.fn("generateGrid")(csp, ctypes.byref(cs), True, True)
+    lib.fn("generateGrid")(ctp, ctypes.byref(ct), True, True)
+    idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int32))
+    res = MU.alloc_bundle("c2Bundle", idx.size)
+    cur = MU.fill_c2bundle(currents, gs)
+    tdir = -1.0 if exp == "fwd" else 1.0
+    t = time.perf_counter()
+    lib.cdll.oracle_farfield_subset(ctypes.byref(res), csp, ctypes.byref(cs), ctypes.byref(ct),
+                                    ctypes.byref(cur), idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                    int(idx.size), float(k), int(nthreads), float(epsilon), tdir)
+    dt = time.perf_counter() - t
+    o = MU.c2bundle_to_obj(res, (idx.size,), "fields")
+    return np.stack(o.components()), dt

@@ -1,0 +1,523 @@
+// abi.hip -- the C ABI of libpypohip.so: drop-in FP64 entry points with the
+// reference's names and argument lists (src/CPU/InterfaceCPU.h:24-64) and the
+// device-resident pypo_hip_* extensions.  Host orchestration only: buffer
+// pool, library stream, error channel, host-side random deviates.
+
+#include <cstring>
+#include <mutex>
+#include <random>
+#include <vector>
+
+#include "common.hpp"
+
+namespace pypo {
+
+// ---- error channel -----------------------------------------------------------
+namespace {
+thread_local int t_err_code = 0;
+thread_local std::string t_err_msg;
+}  // namespace
+
+void set_error(int code, const std::string &msg) {
+    if (t_err_code == 0) {   // keep the first failure of a call
+        t_err_code = code;
+        t_err_msg = msg;
+    }
+}
+
+bool has_error() { return t_err_code != 0; }
+
+// ---- library state ------------------------------------------------------------
+namespace {
+
+std::recursive_mutex g_mu;         // one library call at a time (shared pool)
+void *g_pool[SLOT_COUNT] = {};
+size_t g_pool_bytes[SLOT_COUNT] = {};
+hipStream_t g_stream = nullptr;
+hipEvent_t g_done = nullptr;        // completion of the last enqueued call
+int g_precision = PYPO_PREC_MIXED;
+int g_rt_max_iter = 50;
+bool g_timing = false;
+double g_last_ms = -1.0;
+
+int ensure_init() {
+    if (g_stream) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_error(ERR_NODEV, "no HIP device visible");
+        return ERR_NODEV;
+    }
+    PYPO_HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    PYPO_HIP_TRY(hipEventCreateWithFlags(&g_done, hipEventDisableTiming));
+    return 0;
+}
+
+// Order this call after every earlier call (they share pooled buffers).
+int begin_call(hipStream_t st) {
+    if (g_done) PYPO_HIP_TRY(hipStreamWaitEvent(st, g_done, 0));
+    return 0;
+}
+
+int end_call(hipStream_t st) {
+    PYPO_HIP_TRY(hipEventRecord(g_done, st));
+    return 0;
+}
+
+}  // namespace
+
+void record_kernel_ms(double ms) { g_last_ms = ms; }
+
+hipStream_t lib_stream(int *err) {
+    *err = ensure_init();
+    return g_stream;
+}
+
+void *pool_get(int slot, size_t bytes, int *err) {
+    *err = 0;
+    if (bytes == 0) bytes = 256;
+    if (g_pool_bytes[slot] >= bytes) return g_pool[slot];
+    if (g_pool[slot]) {
+        // earlier asynchronous calls may still read the old buffer
+        (void)hipDeviceSynchronize();
+        (void)hipFree(g_pool[slot]);
+        g_pool[slot] = nullptr;
+        g_pool_bytes[slot] = 0;
+    }
+    size_t want = bytes + bytes / 8;   // headroom for slightly larger calls
+    hipError_t e = hipMalloc(&g_pool[slot], want);
+    if (e != hipSuccess) {
+        g_pool[slot] = nullptr;
+        set_error(ERR_ALLOC, std::string("hipMalloc of ") + std::to_string(want) +
+                                 " bytes failed: " + hipGetErrorString(e));
+        *err = ERR_ALLOC;
+        return nullptr;
+    }
+    g_pool_bytes[slot] = want;
+    return g_pool[slot];
+}
+
+namespace {
+
+// Surface-error deviates of add_surferr_uncorr (InterfaceReflector.cpp:852-866):
+// a new normal_distribution per point, so each draw starts the polar method
+// afresh (Random.h:111-115).
+std::vector<double> grid_deviates(const reflparams &r, int n) {
+    std::vector<double> e(n);
+    std::mt19937 gen(r.rms_seed);
+    for (int i = 0; i < n; i++) {
+        std::normal_distribution<double> d(0.0, r.rms);
+        e[i] = d(gen);
+    }
+    return e;
+}
+
+// Ray surface errors of RayTracer::parallelRays (RayTrace.h:269-274): one
+// distribution for all rays (Random.h:124-137).
+std::vector<double> ray_deviates(const reflparams &r, int n) {
+    std::vector<double> e(n);
+    std::mt19937 gen(r.rms_seed);
+    std::normal_distribution<double> d(0.0, r.rms);
+    for (int i = 0; i < n; i++) e[i] = d(gen);
+    return e;
+}
+
+GridPtrs carve_grid(double *base, int n) {
+    return GridPtrs{base, base + n, base + 2 * (size_t)n, base + 3 * (size_t)n,
+                    base + 4 * (size_t)n, base + 5 * (size_t)n, base + 6 * (size_t)n};
+}
+
+int check_refl(const reflparams &r, int n, const char *what) {
+    if (!r.n_cells || !r.lxu || !r.lyv || !r.gcenter || !r.coeffs || !r.transf) {
+        set_error(ERR_ARG, std::string(what) + ": reflparams has a NULL array");
+        return ERR_ARG;
+    }
+    if ((long long)r.n_cells[0] * r.n_cells[1] != n || n <= 0) {
+        set_error(ERR_ARG, std::string(what) + ": n_cells[0]*n_cells[1] != container size");
+        return ERR_ARG;
+    }
+    return 0;
+}
+
+int check_container(const reflcontainer *c, const char *what) {
+    if (!c || !c->x || !c->y || !c->z || !c->nx || !c->ny || !c->nz || !c->area) {
+        set_error(ERR_ARG, std::string(what) + ": NULL reflcontainer or array");
+        return ERR_ARG;
+    }
+    return 0;
+}
+
+// Grid into device memory (with surface errors when rms > 0).
+int device_grid(const reflparams &r, const GridPtrs &g, int n, int err_slot, bool transform,
+                bool spheric, hipStream_t st) {
+    const double *d_errs = nullptr;
+    if (r.rms > 0) {
+        std::vector<double> e = grid_deviates(r, n);
+        int err = 0;
+        double *d = (double *)pool_get(err_slot, n * sizeof(double), &err);
+        if (err) return err;
+        PYPO_HIP_TRY(hipMemcpyAsync(d, e.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+        PYPO_HIP_TRY(hipStreamSynchronize(st));   // e is a temporary
+        d_errs = d;
+    }
+    return launch_generate_grid(r, g, n, d_errs, transform, spheric, st);
+}
+
+int copy_grid_to_host(const GridPtrs &g, int n, reflcontainer *c, hipStream_t st) {
+    double *src[7] = {g.x, g.y, g.z, g.nx, g.ny, g.nz, g.area};
+    double *dst[7] = {c->x, c->y, c->z, c->nx, c->ny, c->nz, c->area};
+    for (int q = 0; q < 7; q++)
+        PYPO_HIP_TRY(hipMemcpyAsync(dst[q], src[q], n * sizeof(double), hipMemcpyDeviceToHost, st));
+    return 0;
+}
+
+int n_result_arrays(int mode) {
+    switch (mode) {
+        case PYPO_MODE_JMEH: return 24;
+        case PYPO_MODE_EHP: return 15;
+        case PYPO_MODE_SCALAR: return 2;
+        default: return 12;
+    }
+}
+
+// Pointer list of a result/current bundle in field order (struct layout order).
+void bundle_ptrs(int mode, void *b, double **out) {
+    double **p = (double **)b;   // every bundle is a plain sequence of double*
+    int n = n_result_arrays(mode);
+    for (int q = 0; q < n; q++) out[q] = p[q];
+}
+
+// The drop-in PO call: host buffers in, host buffers out, synchronous.
+int po_dropin(int mode, void *res, const reflparams &sp, const reflparams &tp, reflcontainer *cs,
+              reflcontainer *ct, const void *cur, double k, double epsilon, double tdir) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    PYPO_REQUIRE(res && cur, ERR_ARG, "NULL result or source-current bundle");
+    if ((err = check_container(cs, "source")) || (err = check_container(ct, "target"))) return err;
+    const int gs = cs->size, gt = ct->size;
+    if ((err = check_refl(sp, gs, "source")) || (err = check_refl(tp, gt, "target"))) return err;
+    const int nres = n_result_arrays(mode);
+    double *hres[24];
+    bundle_ptrs(mode, res, hres);
+    for (int q = 0; q < nres; q++) PYPO_REQUIRE(hres[q], ERR_ARG, "result bundle has a NULL array");
+    const int ncur = (mode == PYPO_MODE_SCALAR) ? 2 : 12;
+    double *hcur[12];
+    {
+        double **p = (double **)cur;
+        for (int q = 0; q < ncur; q++) {
+            hcur[q] = p[q];
+            PYPO_REQUIRE(hcur[q], ERR_ARG, "source bundle has a NULL array");
+        }
+    }
+    hipStream_t st = g_stream;
+    if ((err = begin_call(st))) return err;
+
+    double *dsg = (double *)pool_get(SLOT_SRC_GRID, 7 * (size_t)gs * sizeof(double), &err);
+    if (err) return err;
+    double *dtg = (double *)pool_get(SLOT_TGT_GRID, 7 * (size_t)gt * sizeof(double), &err);
+    if (err) return err;
+    GridPtrs gsrc = carve_grid(dsg, gs), gtgt = carve_grid(dtg, gt);
+    if ((err = device_grid(sp, gsrc, gs, SLOT_ERRS_S, true, true, st))) return err;
+    if ((err = device_grid(tp, gtgt, gt, SLOT_ERRS_T, true, true, st))) return err;
+
+    double *dcur = (double *)pool_get(SLOT_CURRENTS, ncur * (size_t)gs * sizeof(double), &err);
+    if (err) return err;
+    for (int q = 0; q < ncur; q++)
+        PYPO_HIP_TRY(hipMemcpyAsync(dcur + q * (size_t)gs, hcur[q], gs * sizeof(double),
+                                    hipMemcpyHostToDevice, st));
+
+    const int prec = g_precision;
+    void *packed = pool_get(SLOT_PACKED, packed_bytes(mode, prec, gs), &err);
+    if (err) return err;
+    if (mode == PYPO_MODE_SCALAR) {
+        err = launch_pack_scalar(prec, gsrc, gs, dcur, dcur + gs, sp.gmode, sp.n_cells[0],
+                                 sp.n_cells[1], packed, st);
+    } else {
+        Bundle12 b;
+        for (int q = 0; q < 12; q++) b.p[q] = dcur + q * (size_t)gs;
+        err = launch_pack_sources(prec, gsrc, gs, b, sp.gmode, sp.n_cells[0], sp.n_cells[1], packed,
+                                  st);
+    }
+    if (err) return err;
+
+    SumPlan plan = plan_sum(mode, prec, gs, gt);
+    size_t wsb = partial_bytes(mode, plan, gt);
+    double *ws = (double *)pool_get(SLOT_PARTIAL, wsb, &err);
+    if (err) return err;
+    double *dres = (double *)pool_get(SLOT_RESULT, nres * (size_t)gt * sizeof(double), &err);
+    if (err) return err;
+    double *dptr[24];
+    for (int q = 0; q < nres; q++) dptr[q] = dres + q * (size_t)gt;
+    // device-side bundle with the same layout as the host one
+    c4Bundle dev_b;
+    std::memset(&dev_b, 0, sizeof(dev_b));
+    std::memcpy(&dev_b, dptr, nres * sizeof(double *));
+    err = launch_po(mode, prec, packed, gs, gtgt, gt, k, epsilon, tdir, &dev_b, ws, wsb, st, g_timing);
+    if (err) return err;
+
+    for (int q = 0; q < nres; q++)
+        PYPO_HIP_TRY(hipMemcpyAsync(hres[q], dptr[q], gt * sizeof(double), hipMemcpyDeviceToHost, st));
+    if ((err = copy_grid_to_host(gsrc, gs, cs, st))) return err;
+    if ((err = copy_grid_to_host(gtgt, gt, ct, st))) return err;
+    if ((err = end_call(st))) return err;
+    PYPO_HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // namespace
+}  // namespace pypo
+
+using namespace pypo;
+
+extern "C" {
+
+int pypo_hip_last_error(char *buf, int n) {
+    int code = t_err_code;
+    if (buf && n > 0) {
+        std::snprintf(buf, (size_t)n, "%s", code ? t_err_msg.c_str() : "");
+    }
+    t_err_code = 0;
+    t_err_msg.clear();
+    return code;
+}
+
+const char *pypo_hip_version(void) { return "pypo-hip 0.1 (gfx950)"; }
+
+int pypo_hip_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int pypo_hip_set_precision(int precision) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int old = g_precision;
+    if (precision == PYPO_PREC_MIXED || precision == PYPO_PREC_FP64) g_precision = precision;
+    else set_error(ERR_ARG, "precision must be 0 (mixed) or 1 (fp64)");
+    return old;
+}
+
+int pypo_hip_set_rt_max_iter(int max_iter) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int old = g_rt_max_iter;
+    if (max_iter >= 1) g_rt_max_iter = max_iter;
+    else set_error(ERR_ARG, "max_iter must be >= 1");
+    return old;
+}
+
+int pypo_hip_set_timing(int enable) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int old = g_timing ? 1 : 0;
+    g_timing = enable != 0;
+    g_last_ms = -1.0;
+    return old;
+}
+
+double pypo_hip_last_kernel_ms(void) { return g_last_ms; }
+
+// ---- drop-in PO entry points (InterfaceCPU.h:24-58) ---------------------------
+
+void propagateToGrid_JM(c2Bundle *res, reflparams source, reflparams target, reflcontainer *cs,
+                        reflcontainer *ct, c2Bundle *currents, double k, int numThreads,
+                        double epsilon, double t_direction) {
+    (void)numThreads;
+    po_dropin(PYPO_MODE_JM, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+void propagateToGrid_EH(c2Bundle *res, reflparams source, reflparams target, reflcontainer *cs,
+                        reflcontainer *ct, c2Bundle *currents, double k, int numThreads,
+                        double epsilon, double t_direction) {
+    (void)numThreads;
+    po_dropin(PYPO_MODE_EH, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+void propagateToGrid_JMEH(c4Bundle *res, reflparams source, reflparams target, reflcontainer *cs,
+                          reflcontainer *ct, c2Bundle *currents, double k, int numThreads,
+                          double epsilon, double t_direction) {
+    (void)numThreads;
+    po_dropin(PYPO_MODE_JMEH, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+void propagateToGrid_EHP(c2rBundle *res, reflparams source, reflparams target, reflcontainer *cs,
+                         reflcontainer *ct, c2Bundle *currents, double k, int numThreads,
+                         double epsilon, double t_direction) {
+    (void)numThreads;
+    po_dropin(PYPO_MODE_EHP, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+void propagateToGrid_scalar(arrC1 *res, reflparams source, reflparams target, reflcontainer *cs,
+                            reflcontainer *ct, arrC1 *field, double k, int numThreads,
+                            double epsilon, double t_direction) {
+    (void)numThreads;
+    po_dropin(PYPO_MODE_SCALAR, res, source, target, cs, ct, field, k, epsilon, t_direction);
+}
+
+void propagateToFarField(c2Bundle *res, reflparams source, reflparams target, reflcontainer *cs,
+                         reflcontainer *ct, c2Bundle *currents, double k, int numThreads,
+                         double epsilon, double t_direction) {
+    (void)numThreads;
+    po_dropin(PYPO_MODE_FF, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+// ---- drop-in ray trace (InterfaceCPU.h:60-61) -----------------------------------
+
+void propagateRays(reflparams ctp, cframe *fr_in, cframe *fr_out, int numThreads, double epsilon,
+                   double t0) {
+    (void)numThreads;
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return;
+    if (!fr_in || !fr_out || fr_in->size != fr_out->size || fr_in->size < 0) {
+        set_error(ERR_ARG, "propagateRays: NULL frame or frame sizes differ");
+        return;
+    }
+    if (!ctp.coeffs || !ctp.transf) {
+        set_error(ERR_ARG, "propagateRays: reflparams has a NULL array");
+        return;
+    }
+    const int n = fr_in->size;
+    if (n == 0) return;
+    double *hin[6] = {fr_in->x, fr_in->y, fr_in->z, fr_in->dx, fr_in->dy, fr_in->dz};
+    double *hout[6] = {fr_out->x, fr_out->y, fr_out->z, fr_out->dx, fr_out->dy, fr_out->dz};
+    for (int q = 0; q < 6; q++)
+        if (!hin[q] || !hout[q]) {
+            set_error(ERR_ARG, "propagateRays: frame has a NULL array");
+            return;
+        }
+    hipStream_t st = g_stream;
+    if (begin_call(st)) return;
+    double *d = (double *)pool_get(SLOT_RAYS, 12 * (size_t)n * sizeof(double), &err);
+    if (err) return;
+    for (int q = 0; q < 6; q++)
+        if (hipMemcpyAsync(d + q * (size_t)n, hin[q], n * sizeof(double), hipMemcpyHostToDevice, st) !=
+            hipSuccess) {
+            set_error(ERR_ALLOC, "propagateRays: upload failed");
+            return;
+        }
+    const double *d_errs[1] = {nullptr};
+    if (ctp.rms > 0) {
+        std::vector<double> e = ray_deviates(ctp, n);
+        double *de = (double *)pool_get(SLOT_RT_ERRS, n * sizeof(double), &err);
+        if (err) return;
+        (void)hipMemcpyAsync(de, e.data(), n * sizeof(double), hipMemcpyHostToDevice, st);
+        (void)hipStreamSynchronize(st);
+        d_errs[0] = de;
+    }
+    cframe fr[2];
+    fr[0].size = n; fr[1].size = n;
+    double **f0 = &fr[0].x, **f1 = &fr[1].x;
+    for (int q = 0; q < 6; q++) {
+        f0[q] = d + q * (size_t)n;
+        f1[q] = d + (6 + q) * (size_t)n;
+    }
+    if (launch_trace_chain(&ctp, d_errs, 1, fr, epsilon, t0, g_rt_max_iter, st, g_timing)) return;
+    for (int q = 0; q < 6; q++)
+        (void)hipMemcpyAsync(hout[q], f1[q], n * sizeof(double), hipMemcpyDeviceToHost, st);
+    end_call(st);
+    if (hipStreamSynchronize(st) != hipSuccess) set_error(ERR_ALLOC, "propagateRays: sync failed");
+}
+
+// ---- drop-in grid generation (InterfaceReflector.h:21-22) ---------------------------
+
+void generateGrid(reflparams refl, reflcontainer *container, bool transform, bool spheric) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (ensure_init()) return;
+    if (check_container(container, "generateGrid")) return;
+    const int n = container->size;
+    if (check_refl(refl, n, "generateGrid")) return;
+    hipStream_t st = g_stream;
+    if (begin_call(st)) return;
+    int err = 0;
+    double *d = (double *)pool_get(SLOT_TGT_GRID, 7 * (size_t)n * sizeof(double), &err);
+    if (err) return;
+    GridPtrs g = carve_grid(d, n);
+    if (device_grid(refl, g, n, SLOT_ERRS_T, transform, spheric, st)) return;
+    if (copy_grid_to_host(g, n, container, st)) return;
+    end_call(st);
+    (void)hipStreamSynchronize(st);
+}
+
+// ---- device-resident extensions --------------------------------------------------------
+
+int pypo_hip_generate_grid_dev(reflparams refl, reflcontainer *container, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    if ((err = check_container(container, "grid"))) return err;
+    const int n = container->size;
+    if ((err = check_refl(refl, n, "grid"))) return err;
+    hipStream_t st = stream ? (hipStream_t)stream : g_stream;
+    if ((err = begin_call(st))) return err;
+    if ((err = device_grid(refl, grid_ptrs(*container), n, SLOT_ERRS_S, true, true, st))) return err;
+    return end_call(st);
+}
+
+int pypo_hip_propagate_dev(int mode, void *res, const reflcontainer *cs, const reflcontainer *ct,
+                           const void *currents, int gmode, int ncx, int ncy, double k,
+                           double epsilon, double t_direction, int precision, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    PYPO_REQUIRE(mode >= PYPO_MODE_JM && mode <= PYPO_MODE_SCALAR, ERR_ARG, "unknown PO mode");
+    PYPO_REQUIRE(precision == PYPO_PREC_MIXED || precision == PYPO_PREC_FP64, ERR_ARG,
+                 "unknown precision");
+    PYPO_REQUIRE(res && currents && cs && ct, ERR_ARG, "NULL argument");
+    if ((err = check_container(cs, "source")) || (err = check_container(ct, "target"))) return err;
+    const int gs = cs->size, gt = ct->size;
+    PYPO_REQUIRE(gs > 0 && gt > 0 && (long long)ncx * ncy == gs, ERR_ARG,
+                 "ncx*ncy must equal the source size and both grids must be non-empty");
+    hipStream_t st = stream ? (hipStream_t)stream : g_stream;
+    if ((err = begin_call(st))) return err;
+    void *packed = pool_get(SLOT_PACKED, packed_bytes(mode, precision, gs), &err);
+    if (err) return err;
+    GridPtrs gsrc = grid_ptrs(*cs);
+    if (mode == PYPO_MODE_SCALAR) {
+        const arrC1 *f = (const arrC1 *)currents;
+        err = launch_pack_scalar(precision, gsrc, gs, f->x, f->y, gmode, ncx, ncy, packed, st);
+    } else {
+        err = launch_pack_sources(precision, gsrc, gs, bundle12(*(const c2Bundle *)currents), gmode,
+                                  ncx, ncy, packed, st);
+    }
+    if (err) return err;
+    SumPlan plan = plan_sum(mode, precision, gs, gt);
+    size_t wsb = partial_bytes(mode, plan, gt);
+    double *ws = (double *)pool_get(SLOT_PARTIAL, wsb, &err);
+    if (err) return err;
+    err = launch_po(mode, precision, packed, gs, grid_ptrs(*ct), gt, k, epsilon, t_direction, res, ws,
+                    wsb, st, g_timing);
+    if (err) return err;
+    return end_call(st);
+}
+
+int pypo_hip_trace_dev(const reflparams *surfs, int n_surf, cframe *frames, double tol, double t0,
+                       void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    PYPO_REQUIRE(surfs && frames && n_surf >= 1 && n_surf <= 4, ERR_ARG,
+                 "trace: 1 to 4 surfaces and non-NULL frames");
+    const int n = frames[0].size;
+    hipStream_t st = stream ? (hipStream_t)stream : g_stream;
+    if ((err = begin_call(st))) return err;
+    const double *d_errs[4] = {nullptr, nullptr, nullptr, nullptr};
+    int n_rough = 0;
+    for (int k = 0; k < n_surf; k++) n_rough += surfs[k].rms > 0 ? 1 : 0;
+    if (n_rough && n > 0) {
+        double *de = (double *)pool_get(SLOT_RT_ERRS, (size_t)n_surf * n * sizeof(double), &err);
+        if (err) return err;
+        for (int k = 0; k < n_surf; k++) {
+            if (surfs[k].rms <= 0) continue;
+            std::vector<double> e = ray_deviates(surfs[k], n);
+            PYPO_HIP_TRY(hipMemcpyAsync(de + k * (size_t)n, e.data(), n * sizeof(double),
+                                        hipMemcpyHostToDevice, st));
+            PYPO_HIP_TRY(hipStreamSynchronize(st));
+            d_errs[k] = de + k * (size_t)n;
+        }
+    }
+    if ((err = launch_trace_chain(surfs, d_errs, n_surf, frames, tol, t0, g_rt_max_iter, st,
+                                  g_timing)))
+        return err;
+    return end_call(st);
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+// common.hpp -- shared declarations of libpypohip.so (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/pypo_hip.h"
+
+namespace pypo {
+
+// ---- error channel --------------------------------------------------------
+// Every failure is recorded per calling thread and surfaced by
+// pypo_hip_last_error(); nothing calls exit() or resets the device.
+void set_error(int code, const std::string &msg);
+bool has_error();
+
+#define PYPO_HIP_TRY(expr)                                                        \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) {                                                   \
+            ::pypo::set_error((int)_e, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+            return (int)_e;                                                       \
+        }                                                                         \
+    } while (0)
+
+#define PYPO_CHECK_LAUNCH()                                                       \
+    do {                                                                          \
+        hipError_t _e = hipGetLastError();                                        \
+        if (_e != hipSuccess) {                                                   \
+            ::pypo::set_error((int)_e, std::string("kernel launch: ") + hipGetErrorString(_e)); \
+            return (int)_e;                                                       \
+        }                                                                         \
+    } while (0)
+
+#define PYPO_REQUIRE(cond, code, msg)                                             \
+    do {                                                                          \
+        if (!(cond)) {                                                            \
+            ::pypo::set_error((code), (msg));                                     \
+            return (code);                                                        \
+        }                                                                         \
+    } while (0)
+
+enum { ERR_ARG = 1001, ERR_ALLOC = 1002, ERR_NODEV = 1003 };
+
+// ---- plain-pointer views used inside kernels -------------------------------
+struct GridPtrs {
+    double *x, *y, *z, *nx, *ny, *nz, *area;
+};
+
+inline GridPtrs grid_ptrs(const reflcontainer &c) {
+    return GridPtrs{c.x, c.y, c.z, c.nx, c.ny, c.nz, c.area};
+}
+
+// Complex 3-vector pairs in the c2Bundle order r1x..i2z: index q = 0..11
+// is (re/im, field, axis) as [r1x r1y r1z i1x i1y i1z r2x r2y r2z i2x i2y i2z].
+struct Bundle12 {
+    double *p[12];
+};
+
+inline Bundle12 bundle12(const c2Bundle &b) {
+    return Bundle12{{b.r1x, b.r1y, b.r1z, b.i1x, b.i1y, b.i1z,
+                     b.r2x, b.r2y, b.r2z, b.i2x, b.i2y, b.i2z}};
+}
+
+// ---- packed source records -------------------------------------------------
+// One source point of the PO sums as the kernels read it from HBM/LDS:
+// position in FP64 (the phase and the distance need it), unit normal in
+// FP32 (only the shadowing sign), and the six complex current components
+// pre-multiplied by the trapezoid weight and the area element, in FP32.
+// 96 bytes = six 16-byte LDS reads.
+struct __attribute__((aligned(16))) SrcRec32 {
+    double x, y, z;
+    float nx, ny, nz, pad0;
+    float c[12];   // w*A*(Jx.re, Jx.im, Jy.re, Jy.im, Jz.re, Jz.im, Mx.re, ..., Mz.im)
+    float pad1[2];
+};
+static_assert(sizeof(SrcRec32) == 96, "SrcRec32 must be 96 bytes");
+
+// FP64 twin for the all-FP64 kernels: 6 + 12 doubles = 144 bytes.
+struct __attribute__((aligned(16))) SrcRec64 {
+    double x, y, z;
+    double nx, ny, nz;
+    double c[12];
+};
+static_assert(sizeof(SrcRec64) == 144, "SrcRec64 must be 144 bytes");
+
+// Scalar-mode records: position FP64 and w*A*u (complex) in FP32 / FP64.
+struct __attribute__((aligned(16))) SrcRecS32 {
+    double x, y, z;
+    float ur, ui;
+};
+static_assert(sizeof(SrcRecS32) == 32, "SrcRecS32 must be 32 bytes");
+
+struct __attribute__((aligned(16))) SrcRecS64 {
+    double x, y, z;
+    double ur, ui;
+    double pad;
+};
+static_assert(sizeof(SrcRecS64) == 48, "SrcRecS64 must be 48 bytes");
+
+// ---- device math -----------------------------------------------------------
+
+// sin and cos of 2*pi*f for a reduced argument f in [-0.5, 0.5] revolutions,
+// using the hardware transcendental unit (v_sin_f32 / v_cos_f32 take their
+// argument in revolutions).
+__device__ __forceinline__ void sincos_rev_f32(float f, float &s, float &c) {
+    s = __builtin_amdgcn_sinf(f);
+    c = __builtin_amdgcn_cosf(f);
+}
+
+// FP64 phase in revolutions -> reduced FP32 fraction in [-0.5, 0.5].
+__device__ __forceinline__ float reduce_rev(double rev) {
+    return (float)(rev - __builtin_rint(rev));
+}
+
+// Trapezoid weight of source i on an ncx x ncy grid (Propagation.h:1123-1135:
+// half on the xu edges always, half on the yv edges unless gmode == 1).
+__host__ __device__ __forceinline__ double trapezoid_weight(int i, int gmode, int ncx, int ncy) {
+    int xu = i / ncy, yv = i - xu * ncy;
+    double w = (xu == 0 || xu == ncx - 1) ? 0.5 : 1.0;
+    if (gmode != 1 && (yv == 0 || yv == ncy - 1)) w *= 0.5;
+    return w;
+}
+
+// Physical constants exactly as the reference stores them: float literals
+// widened to double (Propagation.h:196-203).
+struct PhysConsts {
+    double pi_f, prefactor, zeta_inv;
+};
+
+inline PhysConsts phys_consts(double k, double epsilon) {
+    PhysConsts c;
+    c.pi_f = (double)3.14159265359f;
+    double c_l = (double)2.99792458e8f;
+    double mu0 = (double)1.2566370614e-6f;
+    double eps_vac = 1 / (mu0 * c_l * c_l);
+    double eps = epsilon * eps_vac;
+    double zeta = std::sqrt(mu0 / eps);
+    c.zeta_inv = 1 / zeta;
+    c.prefactor = k * k / (4. * c.pi_f);
+    return c;
+}
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+// ---- device buffer pool ----------------------------------------------------
+// Grow-only named device allocations, reused across calls so chained calls
+// do not pay hipMalloc each time.
+void *pool_get(int slot, size_t bytes, int *err);
+enum PoolSlot {
+    SLOT_SRC_GRID = 0, SLOT_TGT_GRID, SLOT_CURRENTS, SLOT_RESULT, SLOT_PACKED,
+    SLOT_PARTIAL, SLOT_ERRS_S, SLOT_ERRS_T, SLOT_RAYS, SLOT_RT_ERRS,
+    SLOT_COUNT
+};
+
+hipStream_t lib_stream(int *err);
+
+// ---- launchers (implemented in the .hip files) -----------------------------
+// r's arrays are host memory (read at launch); g and d_errs are device memory.
+int launch_generate_grid(const reflparams &r, const GridPtrs &g, int n, const double *d_errs,
+                         bool transform, bool spheric, hipStream_t st);
+
+int launch_pack_sources(int precision, const GridPtrs &cs, int n_src, const Bundle12 &cur,
+                        int gmode, int ncx, int ncy, void *packed, hipStream_t st);
+int launch_pack_scalar(int precision, const GridPtrs &cs, int n_src, const double *ur,
+                       const double *ui, int gmode, int ncx, int ncy, void *packed, hipStream_t st);
+size_t packed_bytes(int mode, int precision, int n_src);
+
+// Source-split plan of a PO sum: grid.y = splits, each split sums `chunk`
+// sources (a multiple of the 256-source LDS tile) into its own FP64 partial.
+struct SumPlan {
+    int tpl;          // targets per lane
+    int tblocks;      // workgroups along the targets
+    int splits;       // source splits (grid.y)
+    int chunk;        // sources per split
+};
+SumPlan plan_sum(int mode, int precision, int n_src, int n_tgt);
+size_t partial_bytes(int mode, const SumPlan &p, int n_tgt);
+
+// One PO propagation from packed sources to the target grid ct.  `res`
+// points to a device-side bundle struct of the mode's result type
+// (c2Bundle / c4Bundle / c2rBundle / arrC1) whose arrays are device memory.
+int launch_po(int mode, int precision, const void *packed, int n_src, const GridPtrs &ct,
+              int n_tgt, double k, double epsilon, double t_dir, void *res, double *partial_ws,
+              size_t ws_bytes, hipStream_t st, bool time_it);
+
+// Trace frames[0] through surfs[0..n_surf) (n_surf <= 4); frames[k+1]
+// receives the frame after surface k.  d_errs[k]: per-ray surface error
+// deviates on the device, or nullptr.
+int launch_trace_chain(const reflparams *surfs, const double *const *d_errs, int n_surf,
+                       const cframe *frames, double tol, double t0, int max_iter, hipStream_t st,
+                       bool time_it);
+
+// Kernel timing of the last timed launch (HIP events on the launch stream).
+void record_kernel_ms(double ms);
+
+}  // namespace pypo

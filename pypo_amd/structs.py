@@ -1,0 +1,69 @@
+"""ctypes mirrors of the C ABI structs in include/pypo_hip.h.
+
+Field order and types are those of the reference's src/PyPO/Structs.py:9-141
+(double variants), which mirror src/include/Structs.h:29-161, so a struct
+filled here can be handed to either library.
+"""
+
+import ctypes
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _fields(names):
+    return [(n, _dp) for n in names]
+
+
+_C2 = ["r1x", "r1y", "r1z", "i1x", "i1y", "i1z", "r2x", "r2y", "r2z", "i2x", "i2y", "i2z"]
+_C4 = _C2 + ["r3x", "r3y", "r3z", "i3x", "i3y", "i3z", "r4x", "r4y", "r4z", "i4x", "i4y", "i4z"]
+_C2R = _C2 + ["r3x", "r3y", "r3z"]
+
+
+class arrC1(ctypes.Structure):
+    """Complex scalar field, SoA (Structs.py:12-18)."""
+    _fields_ = _fields(["x", "y"])
+
+
+class c2Bundle(ctypes.Structure):
+    """Two complex 3-vectors per point (Structs.py:29-47)."""
+    _fields_ = _fields(_C2)
+
+
+class c4Bundle(ctypes.Structure):
+    """Four complex 3-vectors per point (Structs.py:49-79)."""
+    _fields_ = _fields(_C4)
+
+
+class c2rBundle(ctypes.Structure):
+    """Two complex and one real 3-vector per point (Structs.py:81-101)."""
+    _fields_ = _fields(_C2R)
+
+
+class reflparams(ctypes.Structure):
+    """Surface parameters, passed by value (Structs.py:97-114)."""
+    _fields_ = [("coeffs", _dp),
+                ("lxu", _dp),
+                ("lyv", _dp),
+                ("n_cells", ctypes.POINTER(ctypes.c_int)),
+                ("flip", ctypes.c_bool),
+                ("gmode", ctypes.c_int),
+                ("gcenter", _dp),
+                ("ecc_uv", ctypes.c_double),
+                ("rot_uv", ctypes.c_double),
+                ("type", ctypes.c_int),
+                ("transf", _dp),
+                ("rms", ctypes.c_double),
+                ("rms_seed", ctypes.c_uint)]
+
+
+class reflcontainer(ctypes.Structure):
+    """Realised grid (Structs.py:116-128)."""
+    _fields_ = [("size", ctypes.c_int)] + _fields(["x", "y", "z", "nx", "ny", "nz", "area"])
+
+
+class cframe(ctypes.Structure):
+    """Ray frame (Structs.py:130-141)."""
+    _fields_ = [("size", ctypes.c_int)] + _fields(["x", "y", "z", "dx", "dy", "dz"])
+
+
+BUNDLE_FIELDS = {"c2Bundle": _C2, "c4Bundle": _C4, "c2rBundle": _C2R, "arrC1": ["x", "y"]}
