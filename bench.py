@@ -1,0 +1,300 @@
+"""Benchmark: PO pair-interactions/s of the large JM -> FF propagation on one GPU.
+
+Workload (the large mixed-precision far-field configuration):
+  dish   paraboloid, D = 3000 mm, f = 1200 mm, uv grid 1001 x 1001 (1 002 001 points)
+  feed   Huygens aperture at the focus, 33 x 33 xy grid over +-1.6 mm, facing the
+         dish, Gaussian amplitude with waist chosen for -12 dB edge illumination
+  far    AoE grid 501 x 501 over +-0.2 deg x +-0.2 deg
+  lambda 1 mm (k = 2 pi / mm)
+One step = JM (feed -> dish currents, near-field kernel) + FF (dish -> far
+field), all buffers resident in HBM, mixed precision (FP32 integrand, FP64
+distance / phase).  value = (JM pairs + FF pairs) / seconds per step.
+
+Also reported: the all-FP64 GPU variant of the same step, the max error of
+both against the FP64 CPU checker on a strided subset of far-field rows, the
+FF kernel's own duration (HIP events on the launch stream) as the roofline
+figure, the ray tracer's rays/s on the Cassegrain configuration, and the
+CPU baseline timed on the host (reference CPU library where it was built,
+else this project's restatement) on the same subset.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fp64] [--no-rt]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+LAM = 1.0
+K = 2 * np.pi / LAM
+F_DISH = 1200.0
+D_DISH = 3000.0
+N_DISH = 1001
+N_FF = 501
+FF_LIM = 0.2
+N_FEED = 33
+FEED_HALF = 1.6
+EDGE_DB = -12.0
+CPU_ROW_STRIDE = 100          # far-field rows 0, 100, ..., 500 for the CPU sample
+
+# Algorithmic work per source-target pair of the far-field sum (FP32 + FP64 flops):
+# 6 complex multiply-adds (48) + shadow test (6) in FP32; r'.r_hat (6) and the
+# phase scale/reduction (4) in FP64; plus one sin and one cos.
+FF_FLOPS_F32 = 54
+FF_FLOPS_F64 = 10
+PEAK_F32 = 157.3e12           # MI355X vector FP32 (AMD published)
+PEAK_F64 = 78.6e12            # MI355X vector FP64 (AMD published)
+PAIR_TIME_FLOOR = FF_FLOPS_F32 / PEAK_F32 + FF_FLOPS_F64 / PEAK_F64   # s per pair at peak
+FF_FLOPS = FF_FLOPS_F32 + FF_FLOPS_F64
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def feed_waist():
+    """Gaussian aperture waist w0 giving EDGE_DB at the dish rim: far-field
+    amplitude exp(-(k w0 sin t / 2)^2) * (1 + cos t) / 2, divided by the focus
+    distance 2f / (1 + cos t), relative to boresight."""
+    te = 2 * np.arctan(D_DISH / (4 * F_DISH))
+    target = 10 ** (EDGE_DB / 20)
+
+    def edge(w0):
+        pat = np.exp(-(K * w0 * np.sin(te) / 2) ** 2) * (1 + np.cos(te)) / 2
+        space = F_DISH / (2 * F_DISH / (1 + np.cos(te)))
+        return pat * space
+
+    lo, hi = 1e-3, 10.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if edge(mid) > target:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def surfaces():
+    from pypo_amd import geometry as G
+    dish = G.paraboloid({"pmode": "focus", "vertex": [0, 0, 0], "focus_1": [0, 0, F_DISH],
+                         "gmode": "uv", "lims_u": [0, D_DISH / 2], "lims_v": [0, 360],
+                         "gridsize": [N_DISH, N_DISH]})
+    feed = G.plane({"gmode": "xy", "lims_x": [-FEED_HALF, FEED_HALF], "lims_y": [-FEED_HALF, FEED_HALF],
+                    "gridsize": [N_FEED, N_FEED], "flip": True})
+    feed["transf"] = G.mat_translate([0, 0, F_DISH])
+    ff = G.plane({"gmode": "AoE", "lims_Az": [-FF_LIM, FF_LIM], "lims_El": [-FF_LIM, FF_LIM],
+                  "gridsize": [N_FF, N_FF]})
+    return feed, dish, ff
+
+
+def feed_currents(w0):
+    from pypo_amd import types as T
+    x = np.linspace(-FEED_HALF, FEED_HALF, N_FEED)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    amp = np.exp(-(X ** 2 + Y ** 2) / w0 ** 2)
+    z = np.zeros_like(amp, dtype=complex)
+    # E || x travelling -z through a surface with normal -z: J = -x, M = +y
+    return T.currents(-amp + 0j, z, z, z, amp + 0j, z)
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def run_rt(torch, D):
+    """Cassegrain ray trace: 3162 x 3162 rays on a 3 m disk, paraboloid then
+    hyperboloid, FP64, tol 1e-12.  Returns rays/s (both surfaces) and ms."""
+    from pypo_amd import geometry as G
+    n1 = 3162
+    g = np.linspace(-D_DISH / 2, D_DISH / 2, n1)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    rng = np.random.default_rng(3)
+    tx = rng.normal(0, 1e-3, X.size)
+    ty = rng.normal(0, 1e-3, X.size)
+    d = np.stack([np.sin(tx), np.sin(ty), -np.ones(X.size)])
+    d /= np.linalg.norm(d, axis=0)
+    fr = np.stack([X.ravel(), Y.ravel(), np.full(X.size, 2000.0), d[0], d[1], d[2]])
+    prim = G.paraboloid({"pmode": "focus", "vertex": [0, 0, 0], "focus_1": [0, 0, F_DISH],
+                         "gmode": "xy", "lims_x": [-1500, 1500], "lims_y": [-1500, 1500],
+                         "gridsize": [3, 3]})
+    sec = G.hyperboloid({"pmode": "focus", "focus_1": [0, 0, F_DISH], "focus_2": [0, 0, -300.0],
+                         "ecc": 1.1, "gmode": "xy", "lims_x": [-150, 150], "lims_y": [-150, 150],
+                         "gridsize": [3, 3]})
+    t_fr = D.to_device(fr)
+    D.trace([prim, sec], t_fr, tol=1e-12, t0=1000.0)
+    torch.cuda.synchronize()
+    reps = 5
+    t = time.perf_counter()
+    for _ in range(reps):
+        D.trace([prim, sec], t_fr, tol=1e-12, t0=1000.0)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    return X.size / dt, dt * 1e3, X.size
+
+
+def cpu_baseline(dish, ff, cur_dish, sub_rows):
+    """FF on the strided row subset with the CPU checker library, timed."""
+    from oracle import bind as OB
+    from pypo_amd import geometry as G
+    kind = "ref" if OB.available("ref") else "port"
+    lib = OB.load(kind)
+    nrow = len(sub_rows)
+    sub = G.plane({"gmode": "AoE", "lims_Az": [-FF_LIM, FF_LIM], "lims_El": [-FF_LIM, FF_LIM],
+                   "gridsize": [nrow, N_FF]})
+    cores = min(os.cpu_count() or 1, int(os.environ.get("PYPO_BENCH_CPU_THREADS", "16")))
+    t = time.perf_counter()
+    out, _, _ = OB.propagate(lib, "FF", dish, sub, cur_dish, K, cores)
+    dt = time.perf_counter() - t
+    vals = np.stack([c.ravel() for c in out.components()])
+    pairs = N_DISH * N_DISH * nrow * N_FF
+    return vals, {"value": pairs / dt, "unit": "pairs/s", "cores": cores,
+                  "kind": "reference" if kind == "ref" else "port",
+                  "sample": f"FF rows {sub_rows[0]}..{sub_rows[-1]} step {CPU_ROW_STRIDE} of the 501x501 "
+                            f"far field ({nrow * N_FF} targets x {N_DISH * N_DISH} sources = {pairs:.3e} "
+                            f"pairs), one native call incl. its grid generation, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-rt", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the PO sums shard over targets with no exchange, "
+              "and this build runs on single-GPU machines; use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pypo_amd import bind as B
+    from pypo_amd import device as D
+
+    lib = B.loadHIPlib()
+    if lib.pypo_hip_device_count() < 1 or not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    w0 = feed_waist()
+    feed, dish, ff = surfaces()
+    log(f"setup: feed waist {w0:.4f} mm, grids on device")
+    g_feed, g_dish, g_ff = D.generate_grid(feed), D.generate_grid(dish), D.generate_grid(ff)
+    c_feed = D.currents_tensor(feed_currents(w0))
+    n_feed, n_dish, n_ff = g_feed.shape[1], g_dish.shape[1], g_ff.shape[1]
+    pairs_jm, pairs_ff = n_feed * n_dish, n_dish * n_ff
+    cur_dish = torch.empty((12, n_dish), dtype=torch.float64, device="cuda")
+    out_ff = torch.empty((12, n_ff), dtype=torch.float64, device="cuda")
+
+    def step(prec):
+        D.propagate("JM", g_feed, g_dish, c_feed, feed, K, precision=prec, out=cur_dish)
+        D.propagate("FF", g_dish, g_ff, cur_dish, dish, K, precision=prec, out=out_ff)
+
+    for _ in range(args.warmup):
+        step("mixed")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        step("mixed")
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / args.steps
+    log(f"mixed step {dt * 1e3:.1f} ms")
+    ff_mixed = D.tensor_to_complex6(out_ff)
+    cur_host = D.tensor_to_complex6(cur_dish)
+
+    # FF kernel alone, timed with HIP events on its stream (roofline figure)
+    lib.pypo_hip_set_timing(1)
+    kms = []
+    for _ in range(3):
+        D.propagate("FF", g_dish, g_ff, cur_dish, dish, K, precision="mixed", out=out_ff)
+        torch.cuda.synchronize()
+        kms.append(lib.pypo_hip_last_kernel_ms())
+    lib.pypo_hip_set_timing(0)
+    ff_ms = float(np.mean(kms))
+    ff_pairs_s = pairs_ff / (ff_ms * 1e-3)
+    achieved = ff_pairs_s * FF_FLOPS / 1e12
+    peak = FF_FLOPS / PAIR_TIME_FLOOR / 1e12
+
+    res = {}
+    ff64 = None
+    if not args.no_fp64:
+        step("fp64")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        step("fp64")
+        torch.cuda.synchronize()
+        dt64 = time.perf_counter() - t
+        ff64 = D.tensor_to_complex6(out_ff)
+        # restore the mixed currents/field for the error check below
+        res["fp64_ms_per_step"] = round(dt64 * 1e3, 2)
+        res["fp64_pairs_per_s"] = (pairs_jm + pairs_ff) / dt64
+        res["mixed_speedup_vs_fp64"] = round(dt64 / dt, 2)
+        res["mixed_vs_fp64_gpu_max_err_rel_peak"] = rel_err(ff_mixed, ff64)
+        log(f"fp64 step {dt64 * 1e3:.1f} ms, mixed vs fp64 err {res['mixed_vs_fp64_gpu_max_err_rel_peak']:.2e}")
+
+    cpu = None
+    if not args.no_cpu:
+        from pypo_amd import types as T
+        rows = list(range(0, N_FF, CPU_ROW_STRIDE))
+        # FP64 checker on the dish currents that the mixed JM step produced
+        cur_obj = T.currents(*[c.reshape(N_DISH, N_DISH) for c in cur_host])
+        log(f"cpu baseline on {len(rows)} far-field rows ...")
+        vals, cpu = cpu_baseline(dish, ff, cur_obj, rows)
+        idx = (np.asarray(rows)[:, None] * N_FF + np.arange(N_FF)[None, :]).ravel()
+        # the checker ran on the mixed-step currents: compare the FF stage
+        D.propagate("FF", g_dish, g_ff, D.currents_tensor(cur_obj), dish, K, precision="mixed",
+                    out=out_ff)
+        ffm = D.tensor_to_complex6(out_ff)[:, idx]
+        res["ff_mixed_max_err_rel_peak_vs_cpu_fp64"] = rel_err(ffm, vals)
+        if not args.no_fp64:
+            D.propagate("FF", g_dish, g_ff, D.currents_tensor(cur_obj), dish, K, precision="fp64",
+                        out=out_ff)
+            ff64s = D.tensor_to_complex6(out_ff)[:, idx]
+            res["ff_fp64_max_err_rel_peak_vs_cpu_fp64"] = rel_err(ff64s, vals)
+        torch.cuda.synchronize()
+        res["speedup_vs_cpu_ff"] = round(ff_pairs_s / cpu["value"], 1)
+        log(f"cpu {cpu['value']:.3e} pairs/s; errors {res}")
+
+    if not args.no_rt:
+        rays_s, rt_ms, nrays = run_rt(torch, D)
+        res["rt_rays_per_s"] = rays_s
+        res["rt_ms_cassegrain"] = round(rt_ms, 3)
+        res["rt_hbm_GBps"] = round(nrays * (48 + 2 * 48) / (rt_ms * 1e-3) / 1e9, 1)
+        log(f"rt {rays_s:.3e} rays/s ({rt_ms:.2f} ms for {nrays} rays x 2 surfaces)")
+
+    line = {
+        "metric": "PO pair-interactions/s JM->FF (mixed FP32/FP64 vs FP64) at <=1e-5 peak err; rays/s",
+        "value": (pairs_jm + pairs_ff) / dt,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic (seedless analytic Gaussian feed currents; grids generated on device)",
+        "config": {"workload": "JM->FF large far field: feed 33x33 -> paraboloid D=3000 f=1200 uv "
+                               "1001x1001 -> AoE 501x501 +-0.2deg, lambda=1mm",
+                   "pairs_per_step": pairs_jm + pairs_ff, "jm_pairs": pairs_jm, "ff_pairs": pairs_ff,
+                   "feed_waist_mm": round(w0, 5), "precision": "mixed"},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": round(peak, 1),
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel": "k_ff_mixed", "kernel_ms": round(ff_ms, 3),
+                     "work_per_pair": f"{FF_FLOPS_F32} FP32 + {FF_FLOPS_F64} FP64 flops + sincos"},
+        "cpu_baseline": cpu,
+    }
+    line.update(res)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -174,8 +174,10 @@ int pypo_hip_set_rt_max_iter(int max_iter);
  * the scalar mode).  res is c2Bundle (JM, EH, FF), c4Bundle (JMEH),
  * c2rBundle (EHP) or arrC1 (scalar) according to mode.  gmode/ncx/ncy are
  * the source grid's gmode and n_cells (trapezoid weights,
- * Propagation.h:1123-1135).  stream may be NULL (library stream).
- * Asynchronous on `stream`.  Returns 0 on success. */
+ * Propagation.h:1123-1135).  stream is a hipStream_t; NULL means the
+ * default (null) stream.  Asynchronous on `stream`; calls are ordered after
+ * every earlier library call whatever their streams (they share the
+ * library's workspace).  Returns 0 on success. */
 int pypo_hip_propagate_dev(int mode, void *res, const reflcontainer *cs,
                            const reflcontainer *ct, const void *currents,
                            int gmode, int ncx, int ncy, double k, double epsilon,

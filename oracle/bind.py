@@ -81,7 +81,9 @@ def load(kind="port"):
 
 
 def default_threads():
-    return max(1, os.cpu_count() or 1)
+    """Host threads for the checker: the CPU share of one GPU (16), or fewer
+    when the machine has fewer cores."""
+    return max(1, min(16, os.cpu_count() or 1))
 
 
 def generate_grid(lib, refl, transform=True, spheric=True):

@@ -42,6 +42,15 @@ def loadHIPlib():
     with _lock:
         if _lib is not None:
             return _lib
+        # One HIP runtime per process: PyTorch-ROCm carries its own
+        # libamdhip64.so.7 / libhsa-runtime64.so.1.  Loading torch first makes
+        # libpypohip.so bind to that same runtime (matching sonames); loading
+        # this library first would leave torch's HIP on a foreign HSA runtime
+        # with no visible GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise OSError(f"{LIB_PATH} is missing: build it with "
                           "`python -c 'import __graft_entry__ as g; g.build()'`")

@@ -446,7 +446,7 @@ int pypo_hip_generate_grid_dev(reflparams refl, reflcontainer *container, void *
     if ((err = check_container(container, "grid"))) return err;
     const int n = container->size;
     if ((err = check_refl(refl, n, "grid"))) return err;
-    hipStream_t st = stream ? (hipStream_t)stream : g_stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the default stream
     if ((err = begin_call(st))) return err;
     if ((err = device_grid(refl, grid_ptrs(*container), n, SLOT_ERRS_S, true, true, st))) return err;
     return end_call(st);
@@ -466,7 +466,7 @@ int pypo_hip_propagate_dev(int mode, void *res, const reflcontainer *cs, const r
     const int gs = cs->size, gt = ct->size;
     PYPO_REQUIRE(gs > 0 && gt > 0 && (long long)ncx * ncy == gs, ERR_ARG,
                  "ncx*ncy must equal the source size and both grids must be non-empty");
-    hipStream_t st = stream ? (hipStream_t)stream : g_stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the default stream
     if ((err = begin_call(st))) return err;
     void *packed = pool_get(SLOT_PACKED, packed_bytes(mode, precision, gs), &err);
     if (err) return err;
@@ -497,7 +497,7 @@ int pypo_hip_trace_dev(const reflparams *surfs, int n_surf, cframe *frames, doub
     PYPO_REQUIRE(surfs && frames && n_surf >= 1 && n_surf <= 4, ERR_ARG,
                  "trace: 1 to 4 surfaces and non-NULL frames");
     const int n = frames[0].size;
-    hipStream_t st = stream ? (hipStream_t)stream : g_stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the default stream
     if ((err = begin_call(st))) return err;
     const double *d_errs[4] = {nullptr, nullptr, nullptr, nullptr};
     int n_rough = 0;

@@ -1,0 +1,85 @@
+"""C-ABI checks that need no GPU: the library builds, loads, exports every
+entry point include/pypo_hip.h declares, and fails loudly (never silently,
+never through a CPU fallback) when no device is visible."""
+
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from pypo_amd import bind as B
+from pypo_amd import geometry as G
+from pypo_amd import structs as PS
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pypo_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\*?\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_the_reference_entry_points():
+    names = declared_functions()
+    # InterfaceCPU.h:24-61 FP64 symbol set, unchanged names
+    for ref_name in ("propagateToGrid_JM", "propagateToGrid_EH", "propagateToGrid_JMEH",
+                     "propagateToGrid_EHP", "propagateToGrid_scalar", "propagateToFarField",
+                     "propagateRays", "generateGrid"):
+        assert ref_name in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(B.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", B.LIB_PATH], capture_output=True, text=True).stdout
+    for name in declared_functions():
+        assert re.search(rf"\bT {name}$", out, flags=re.M), name
+
+
+def test_struct_layouts_match_the_reference_abi():
+    # x86-64 SysV sizes of src/include/Structs.h:29-161
+    assert ctypes.sizeof(PS.c2Bundle) == 12 * 8
+    assert ctypes.sizeof(PS.c4Bundle) == 24 * 8
+    assert ctypes.sizeof(PS.c2rBundle) == 15 * 8
+    assert ctypes.sizeof(PS.reflcontainer) == 8 + 7 * 8
+    assert ctypes.sizeof(PS.cframe) == 8 + 6 * 8
+    assert PS.reflparams.flip.offset == 32 and PS.reflparams.gmode.offset == 36
+    assert PS.reflparams.transf.offset == 72 and PS.reflparams.rms_seed.offset == 88
+    assert ctypes.sizeof(PS.reflparams) == 96
+
+
+def test_version_and_device_count_callable():
+    lib = B.loadHIPlib()
+    assert b"gfx950" in lib.pypo_hip_version()
+    assert lib.pypo_hip_device_count() >= 0
+
+
+def test_no_device_raises_instead_of_falling_back():
+    lib = B.loadHIPlib()
+    if lib.pypo_hip_device_count() > 0:
+        pytest.skip("a device is visible here")
+    src = G.plane({"gmode": "xy", "lims_x": [-1, 1], "lims_y": [-1, 1], "gridsize": [3, 3]})
+    tgt = G.plane({"gmode": "xy", "lims_x": [-1, 1], "lims_y": [-1, 1], "gridsize": [3, 3]})
+    from pypo_amd import types as T
+    z = np.zeros((3, 3), complex)
+    with pytest.raises(B.PyPOHIPError, match="no HIP device"):
+        B.PyPO_HIPd(src, tgt, {"mode": "EH", "k": 1.0, "s_current": T.currents(z, z, z, z, z, z)})
+
+
+def test_argument_errors_are_reported_not_fatal():
+    lib = B.loadHIPlib()
+    buf = ctypes.create_string_buffer(256)
+    lib.pypo_hip_last_error(buf, 256)           # clear
+    lib.pypo_hip_set_precision(7)
+    assert lib.pypo_hip_last_error(buf, 256) != 0
+    assert b"precision" in buf.value
+    assert lib.pypo_hip_last_error(buf, 256) == 0    # cleared after reading
+    lib.pypo_hip_set_rt_max_iter(0)
+    assert lib.pypo_hip_last_error(buf, 256) != 0

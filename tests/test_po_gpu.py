@@ -1,0 +1,186 @@
+"""Parity of the HIP PO kernels (through the C ABI) with the CPU checker.
+
+Bars: FP64 path <= 1e-10 of each field's peak; mixed path (FP32 integrand,
+FP64 distance/phase) <= 1e-5 of each field's peak -- the tolerance the
+north-star metric is quoted at.  Peaks are taken per field (E, H, J, M,
+S_r separately).
+"""
+
+import numpy as np
+import pytest
+
+import cases as C
+from helpers import flat, group_err
+from oracle import bind as OB
+from pypo_amd import bind as B
+from pypo_amd import geometry as G
+from pypo_amd import types as T
+from pypo_amd.geometry import gridsize
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-10, "mixed": 1e-5}
+
+
+def _src_for(c):
+    shape = gridsize(c["source"])
+    return (C.random_scalar(shape, c["seed"]) if c["mode"] == "scalar"
+            else C.random_currents(shape, c["seed"]))
+
+
+def _run(c, prec):
+    d = {"mode": c["mode"], "k": 2 * np.pi / c["lam"], "exp": c["exp"], "precision": prec,
+         "epsilon": c.get("epsilon", 1.0)}
+    d["s_scalarfield" if c["mode"] == "scalar" else "s_current"] = _src_for(c)
+    return B.PyPO_HIPd(c["source"], c["target"], d, return_grids=True)
+
+
+@pytest.fixture(scope="module")
+def gold_po():
+    return np.load("tests/golden/po.npz")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", sorted(C.po_cases()))
+def test_po_matches_golden(hip, gold_po, name, prec):
+    c = C.po_cases()[name]
+    out, gs, gt = _run(c, prec)
+    err = group_err(flat(out), gold_po[f"{name}/out"])
+    print(f"{name} {prec}: {err:.2e}")
+    assert err <= TOL[prec]
+    # the drop-in writes the generated target grid back like the reference
+    tg = np.stack([np.ravel(getattr(gt, a)) for a in ("x", "y", "z", "nx", "ny", "nz", "area")])
+    np.testing.assert_allclose(tg, gold_po[f"{name}/tgt_grid"], rtol=1e-13, atol=1e-12)
+
+
+def _cfg1():
+    """Plain PO configuration: vector Gaussian-like feed, lambda = 3 mm, on a
+    101 x 101 plane over +-20 mm at the focus of a paraboloid (f = 0.5 m,
+    D = 0.3 m, uv 151 x 151); JM on the dish, then FF onto 101 x 101 AoE +-1 deg."""
+    lam = 3.0
+    feed = G.plane({"gmode": "xy", "lims_x": [-20, 20], "lims_y": [-20, 20], "gridsize": [101, 101],
+                    "flip": True})
+    feed["transf"] = G.mat_translate([0, 0, 500.0])
+    dish = G.paraboloid({"pmode": "focus", "vertex": [0, 0, 0], "focus_1": [0, 0, 500.0], "gmode": "uv",
+                         "lims_u": [0, 150], "lims_v": [0, 360], "gridsize": [151, 151]})
+    ff = G.plane({"gmode": "AoE", "lims_Az": [-1, 1], "lims_El": [-1, 1], "gridsize": [101, 101]})
+    x = np.linspace(-20, 20, 101)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    amp = np.exp(-(X ** 2 + Y ** 2) / 5.0 ** 2)
+    z = np.zeros_like(amp, dtype=complex)
+    cur = T.currents(-amp + 0j, z, z, z, amp + 0j, z)
+    return lam, feed, dish, ff, cur
+
+
+@pytest.fixture(scope="module")
+def cfg1_ref(port):
+    lam, feed, dish, ff, cur = _cfg1()
+    k = 2 * np.pi / lam
+    jm, _, _ = OB.propagate(port, "JM", feed, dish, cur, k)
+    eh, _, _ = OB.propagate(port, "FF", dish, ff, jm, k)
+    return jm, eh
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_cfg1_jm_then_ff(hip, cfg1_ref, prec):
+    lam, feed, dish, ff, cur = _cfg1()
+    k = 2 * np.pi / lam
+    jm_ref, eh_ref = cfg1_ref
+    jm = B.PyPO_HIPd(feed, dish, {"mode": "JM", "k": k, "s_current": cur, "precision": prec})
+    e1 = group_err(flat(jm), flat(jm_ref))
+    eh = B.PyPO_HIPd(dish, ff, {"mode": "FF", "k": k, "s_current": jm_ref, "precision": prec})
+    e2 = group_err(flat(eh), flat(eh_ref))
+    print(f"cfg1 {prec}: JM {e1:.2e} FF {e2:.2e}")
+    assert e1 <= TOL[prec] and e2 <= TOL[prec]
+
+
+def test_linearity_and_device_chaining(hip):
+    import torch
+    from pypo_amd import device as D
+    lam, feed, dish, ff, cur = _cfg1()
+    k = 2 * np.pi / lam
+    g_feed, g_dish, g_ff = D.generate_grid(feed), D.generate_grid(dish), D.generate_grid(ff)
+    c1 = D.currents_tensor(cur)
+    rng = np.random.default_rng(3)
+    c2 = torch.as_tensor(rng.standard_normal(c1.shape) * 0.1, device="cuda")
+    jm = D.propagate("JM", g_feed, g_dish, c1, feed, k, precision="fp64")
+    ffa = D.propagate("FF", g_dish, g_ff, jm, dish, k, precision="mixed")
+    ffb = D.propagate("FF", g_dish, g_ff, jm * 2.5, dish, k, precision="mixed")
+    ffc = D.propagate("FF", g_dish, g_ff, jm + 0, dish, k, precision="mixed")
+    torch.cuda.synchronize()
+    # a zero-copy chain gives the same field as the host drop-in path
+    jm_host = B.PyPO_HIPd(feed, dish, {"mode": "JM", "k": k, "s_current": cur, "precision": "fp64"})
+    eh_host = B.PyPO_HIPd(dish, ff, {"mode": "FF", "k": k, "s_current": jm_host, "precision": "mixed"})
+    assert group_err(D.tensor_to_complex6(ffa), flat(eh_host)) < 1e-12
+    # deterministic: the same inputs give bit-identical results
+    assert torch.equal(ffa, ffc)
+    # linear in the currents (scaling by 2.5 is exact in FP32 up to rounding)
+    assert group_err(D.tensor_to_complex6(ffb), 2.5 * D.tensor_to_complex6(ffa)) < 1e-6
+    # linear in superposition, mixed precision
+    jm2 = D.propagate("JM", g_feed, g_dish, c2, feed, k, precision="fp64")
+    ffs = D.propagate("FF", g_dish, g_ff, jm + jm2, dish, k, precision="fp64")
+    ff2 = D.propagate("FF", g_dish, g_ff, jm2, dish, k, precision="fp64")
+    ff1 = D.propagate("FF", g_dish, g_ff, jm, dish, k, precision="fp64")
+    torch.cuda.synchronize()
+    assert group_err(D.tensor_to_complex6(ffs), D.tensor_to_complex6(ff1 + ff2)) < 1e-12
+
+
+def test_large_far_field_mixed_vs_fp64_and_transverse(hip, port):
+    """The large configuration at full size (1001 x 1001 dish -> 501 x 501):
+    the mixed kernel against the FP64 kernel on every target, E and H
+    transverse to r_hat, and the FP64 kernel against the CPU checker on one
+    far-field row (501 targets x 1e6 sources)."""
+    import torch
+    import bench
+    from pypo_amd import device as D
+    w0 = bench.feed_waist()
+    feed, dish, ff = bench.surfaces()
+    g_feed, g_dish, g_ff = D.generate_grid(feed), D.generate_grid(dish), D.generate_grid(ff)
+    c_feed = D.currents_tensor(bench.feed_currents(w0))
+    jm = D.propagate("JM", g_feed, g_dish, c_feed, feed, bench.K, precision="fp64")
+    m = D.propagate("FF", g_dish, g_ff, jm, dish, bench.K, precision="mixed")
+    f = D.propagate("FF", g_dish, g_ff, jm, dish, bench.K, precision="fp64")
+    torch.cuda.synchronize()
+    M, F = D.tensor_to_complex6(m), D.tensor_to_complex6(f)
+    err = group_err(M, F)
+    print(f"large FF mixed vs fp64: {err:.2e}")
+    assert err <= 1e-5
+    gf = g_ff.cpu().numpy()
+    th, ph = gf[0], gf[1]
+    r = np.stack([np.cos(ph) * np.sin(th), np.sin(ph) * np.sin(th), np.cos(th)])
+    for g in (0, 3):
+        assert np.max(np.abs(np.sum(M[g:g + 3] * r, axis=0))) < 1e-6 * np.max(np.abs(M[g:g + 3]))
+    # one far-field row against the FP64 CPU checker
+    row = 250
+    idx = row * bench.N_FF + np.arange(bench.N_FF)
+    cur = T.currents(*[c.reshape(bench.N_DISH, bench.N_DISH) for c in D.tensor_to_complex6(jm)])
+    vals, _ = OB.farfield_subset(port, dish, ff, cur, bench.K, idx)
+    e64 = group_err(F[:, idx], vals)
+    em = group_err(M[:, idx], vals)
+    print(f"large FF row {row}: fp64 {e64:.2e} mixed {em:.2e} vs CPU FP64")
+    assert e64 <= 1e-10 and em <= 1e-5
+
+
+def test_bad_arguments_raise(hip):
+    src = G.plane({"gmode": "xy", "lims_x": [-1, 1], "lims_y": [-1, 1], "gridsize": [3, 3]})
+    z = np.zeros((3, 3), complex)
+    cur = T.currents(z, z, z, z, z, z)
+    with pytest.raises(B.PyPOHIPError):
+        B.PyPO_HIPd(src, dict(src, gridsize=np.array([0, 3])), {"mode": "EH", "k": 1.0, "s_current": cur})
+    with pytest.raises(B.PyPOHIPError):
+        B.PyPO_HIPd(src, src, {"mode": "XX", "k": 1.0, "s_current": cur})
+    # the library stays usable after an error
+    out = B.PyPO_HIPd(src, dict(src, transf=G.mat_translate([0, 0, 5.0])),
+                      {"mode": "EH", "k": 1.0, "s_current": cur})
+    assert np.all(flat(out) == 0)
+
+
+def test_kernel_timing_api(hip):
+    lib = hip
+    c = C.po_cases()["FF_uv_fwd"]
+    lib.pypo_hip_set_timing(1)
+    try:
+        _run(c, "mixed")
+        assert lib.pypo_hip_last_kernel_ms() > 0
+    finally:
+        lib.pypo_hip_set_timing(0)

@@ -1,0 +1,106 @@
+"""Parity of the HIP ray tracer and grid generator with the CPU checker.
+
+Bar for hit points: <= 1e-12 m = 1e-9 mm; directions <= 1e-12."""
+
+import numpy as np
+import pytest
+
+import cases as C
+from oracle import bind as OB
+from pypo_amd import bind as B
+from pypo_amd import geometry as G
+from pypo_amd import types as T
+
+pytestmark = pytest.mark.gpu
+
+COMP = ("x", "y", "z", "dx", "dy", "dz")
+
+
+def _stack(fr):
+    return np.stack([getattr(fr, a) for a in COMP])
+
+
+def test_rt_matches_golden(hip):
+    gold = np.load("tests/golden/rt.npz")
+    for name, c in C.rt_cases().items():
+        fr = C.rt_frame(c)
+        x0 = fr.x.copy()
+        out = B.RT_HIPd({"fr_in": fr, "t_name": c["target"], "tol": c["tol"], "t0": c["t0"]})
+        got, want = _stack(out), gold[f"{name}/out"]
+        print(name, np.max(np.abs(got[:3] - want[:3])), np.max(np.abs(got[3:] - want[3:])))
+        np.testing.assert_allclose(got[:3], want[:3], rtol=0, atol=1e-9, err_msg=name)
+        np.testing.assert_allclose(got[3:], want[3:], rtol=0, atol=1e-12, err_msg=name)
+        np.testing.assert_array_equal(fr.x, x0)    # input frame untouched
+
+
+def _cassegrain(n1=3162):
+    import bench
+    g = np.linspace(-bench.D_DISH / 2, bench.D_DISH / 2, n1)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    rng = np.random.default_rng(3)
+    tx, ty = rng.normal(0, 1e-3, X.size), rng.normal(0, 1e-3, X.size)
+    d = np.stack([np.sin(tx), np.sin(ty), -np.ones(X.size)])
+    d /= np.linalg.norm(d, axis=0)
+    fr = T.frame(X.size, X.ravel(), Y.ravel(), np.full(X.size, 2000.0), d[0], d[1], d[2])
+    prim = G.paraboloid({"pmode": "focus", "vertex": [0, 0, 0], "focus_1": [0, 0, bench.F_DISH],
+                         "gmode": "xy", "lims_x": [-1500, 1500], "lims_y": [-1500, 1500], "gridsize": [3, 3]})
+    sec = G.hyperboloid({"pmode": "focus", "focus_1": [0, 0, bench.F_DISH], "focus_2": [0, 0, -300.0],
+                         "ecc": 1.1, "gmode": "xy", "lims_x": [-150, 150], "lims_y": [-150, 150],
+                         "gridsize": [3, 3]})
+    return fr, prim, sec
+
+
+def test_cassegrain_full_size_against_cpu(hip, port):
+    """1e7 rays (3162 x 3162 over the 3 m disk, N(0, 1e-3 rad) tilts) through
+    the paraboloid and the hyperboloid, FP64, tol 1e-12, both surfaces in one
+    device launch, against the CPU checker surface by surface."""
+    import torch
+    from pypo_amd import device as D
+    fr, prim, sec = _cassegrain()
+    t_fr = D.to_device(np.stack([getattr(fr, a) for a in COMP]))
+    f1, f2 = D.trace([prim, sec], t_fr, tol=1e-12, t0=1000.0)
+    torch.cuda.synchronize()
+    g1, g2 = f1.cpu().numpy(), f2.cpu().numpy()
+    r1 = OB.trace(port, prim, fr, None, 1e-12, 1000.0, max_iter=50)
+    r2 = OB.trace(port, sec, r1, None, 1e-12, 1000.0, max_iter=50)
+    e1 = np.max(np.abs(g1[:3] - _stack(r1)[:3]))
+    e2 = np.max(np.abs(g2[:3] - _stack(r2)[:3]))
+    d2 = np.max(np.abs(g2[3:] - _stack(r2)[3:]))
+    print(f"cassegrain hit-point max diff: primary {e1:.2e} mm, secondary {e2:.2e} mm, dir {d2:.2e}")
+    assert e1 <= 1e-9 and e2 <= 1e-9 and d2 <= 1e-12
+    # the chain equals two separate drop-in calls
+    one = B.RT_HIPd({"fr_in": fr, "t_name": prim, "tol": 1e-12, "t0": 1000.0})
+    np.testing.assert_array_equal(_stack(one), g1)
+    # axis-parallel rays converge on the secondary focus (0, 0, -300)
+    x, y = fr.x[::97], fr.y[::97]
+    n = x.size
+    par = D.to_device(np.stack([x, y, np.full(n, 2000.0), np.zeros(n), np.zeros(n), -np.ones(n)]))
+    _, h2 = D.trace([prim, sec], par, tol=1e-12, t0=1000.0)
+    h2 = h2.cpu().numpy()
+    s = (-300.0 - h2[2]) / h2[5]
+    assert np.max(np.abs(h2[0] + s * h2[3])) < 1e-6
+    assert np.max(np.abs(h2[1] + s * h2[4])) < 1e-6
+
+
+def test_degenerate_rays_follow_the_miss_rule(hip, port):
+    """A ray parallel to a plane (dz = 0) drives the Newton update to inf and
+    then NaN: the loop ends and the reference's miss rule applies (position
+    copied, direction zeroed), within the iteration cap."""
+    pla = G.plane({"gmode": "xy", "lims_x": [-1, 1], "lims_y": [-1, 1], "gridsize": [3, 3]})
+    n = 5
+    fr = T.frame(n, np.arange(n, dtype=float), np.zeros(n), np.ones(n), np.ones(n), np.zeros(n),
+                 np.zeros(n))
+    out = B.RT_HIPd({"fr_in": fr, "t_name": pla, "tol": 1e-12, "t0": 1.0}, max_iter=7)
+    ref = OB.trace(port, pla, fr, 1, 1e-12, 1.0, max_iter=7)
+    np.testing.assert_array_equal(np.isnan(_stack(out)), np.isnan(_stack(ref)))
+    np.testing.assert_allclose(_stack(out), _stack(ref), atol=1e-12)
+
+
+def test_grids_match_golden(hip):
+    gold = np.load("tests/golden/grids.npz")
+    for name, refl in C.grid_cases().items():
+        for tr, sp in ((True, True), (False, False)):
+            g = B.generate_grid(refl, tr, sp)
+            got = np.stack([np.ravel(getattr(g, a)) for a in ("x", "y", "z", "nx", "ny", "nz", "area")])
+            np.testing.assert_allclose(got, gold[f"{name}/{int(tr)}{int(sp)}"], rtol=1e-13, atol=1e-12,
+                                       err_msg=name)
