@@ -71,14 +71,26 @@ inline Bundle12 bundle12(const c2Bundle &b) {
 // position in FP64 (the phase and the distance need it), unit normal in
 // FP32 (only the shadowing sign), and the six complex current components
 // pre-multiplied by the trapezoid weight and the area element, in FP32.
-// 96 bytes = six 16-byte LDS reads.
+// 96 bytes; the J-only, unmasked far-field loop reads bytes 0-23 and 40-63.
 struct __attribute__((aligned(16))) SrcRec32 {
-    double x, y, z;
-    float nx, ny, nz, pad0;
-    float c[12];   // w*A*(Jx.re, Jx.im, Jy.re, Jy.im, Jz.re, Jz.im, Mx.re, ..., Mz.im)
-    float pad1[2];
+    double x, y, z;   // 0..23
+    float nx, ny;     // 24..31
+    float nz, pad0;   // 32..39
+    float cj[6];      // 40..63  w*A*(Jx.re, Jx.im, Jy.re, Jy.im, Jz.re, Jz.im)
+    float cm[6];      // 64..87  w*A*(Mx.re, ..., Mz.im)
+    float pad1[2];    // 88..95
 };
 static_assert(sizeof(SrcRec32) == 96, "SrcRec32 must be 96 bytes");
+
+// Per 256-source tile: the cone that holds the tile's normals (unit axis and
+// the cosine of its half-angle; -1 when no cone under 90 degrees exists) and
+// whether any magnetic current in the tile is non-zero.  Lets the far-field
+// sum skip the per-pair shadow test and the M terms where they cannot matter.
+struct __attribute__((aligned(16))) TileMeta {
+    float ax, ay, az, cos_half;
+    int has_m, n_valid, pad0, pad1;
+};
+static_assert(sizeof(TileMeta) == 32, "TileMeta must be 32 bytes");
 
 // FP64 twin for the all-FP64 kernels: 6 + 12 doubles = 144 bytes.
 struct __attribute__((aligned(16))) SrcRec64 {
@@ -168,6 +180,7 @@ int launch_pack_sources(int precision, const GridPtrs &cs, int n_src, const Bund
                         int gmode, int ncx, int ncy, void *packed, hipStream_t st);
 int launch_pack_scalar(int precision, const GridPtrs &cs, int n_src, const double *ur,
                        const double *ui, int gmode, int ncx, int ncy, void *packed, hipStream_t st);
+// Packed records plus (for the vector modes) one TileMeta per 256 sources.
 size_t packed_bytes(int mode, int precision, int n_src);
 
 // Source-split plan of a PO sum: grid.y = splits, each split sums `chunk`

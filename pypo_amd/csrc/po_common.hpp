@@ -1,0 +1,162 @@
+// po_common.hpp -- device helpers shared by the PO sum kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace pypo {
+
+constexpr int TILE = 256;   // sources per LDS tile == threads per workgroup
+
+// LDS tile staging: a tile of TILE records of REC_U4 16-byte words, loaded
+// one record per lane.  Past the end of the split the lane writes a null
+// record instead (all weights zero, position 1e7 mm away from anything) so
+// that the inner loop always runs TILE iterations and a null record can
+// never produce a zero distance.
+template <int REC_U4>
+struct TileLoader {
+    uint4 buf[REC_U4];
+    __device__ __forceinline__ void load(const uint4 *__restrict__ g, int s, int s_end) {
+        if (s < s_end) {
+#pragma unroll
+            for (int k = 0; k < REC_U4; k++) buf[k] = g[(size_t)s * REC_U4 + k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < REC_U4; k++) buf[k] = make_uint4(0u, 0u, 0u, 0u);
+            // every record starts with double x, y, z
+            const unsigned long long far = (unsigned long long)__double_as_longlong(1.0e7);
+            const unsigned lo = (unsigned)(far & 0xffffffffull), hi = (unsigned)(far >> 32);
+            buf[0] = make_uint4(lo, hi, lo, hi);
+            buf[1].x = lo;
+            buf[1].y = hi;
+        }
+    }
+    __device__ __forceinline__ void store(uint4 *lds, int tid) {
+#pragma unroll
+        for (int k = 0; k < REC_U4; k++) lds[tid * REC_U4 + k] = buf[k];
+    }
+};
+
+// Wave-level reductions (64 lanes) through cross-lane shuffles.
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Workgroup (256 lanes = 4 waves) reductions; every lane gets the result.
+// `scratch` holds at least 4 floats of LDS; callers separate uses with the
+// barriers inside.
+__device__ __forceinline__ float block_sum(float v, float *scratch) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) scratch[w] = v;
+    __syncthreads();
+    return scratch[0] + scratch[1] + scratch[2] + scratch[3];
+}
+
+__device__ __forceinline__ float block_min(float v, float *scratch) {
+    v = wave_min(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) scratch[w] = v;
+    __syncthreads();
+    return fminf(fminf(scratch[0], scratch[1]), fminf(scratch[2], scratch[3]));
+}
+
+// Cone of directions: unit axis and cos of the half-angle (-1 if none).
+struct Cone {
+    float ax, ay, az, cos_half;
+};
+
+// The cone holding the unit vectors (vx, vy, vz) of the lanes with
+// valid == true, over the workgroup.
+__device__ __forceinline__ Cone block_cone(float vx, float vy, float vz, bool valid, float *scratch) {
+    float sx = block_sum(valid ? vx : 0.f, scratch);
+    float sy = block_sum(valid ? vy : 0.f, scratch);
+    float sz = block_sum(valid ? vz : 0.f, scratch);
+    float cnt = block_sum(valid ? 1.f : 0.f, scratch);
+    float len = sqrtf(sx * sx + sy * sy + sz * sz);
+    Cone c;
+    if (!(len > 1e-4f * cnt) || cnt == 0.f) {
+        c.ax = 0.f; c.ay = 0.f; c.az = 1.f; c.cos_half = -1.f;
+        return c;
+    }
+    c.ax = sx / len; c.ay = sy / len; c.az = sz / len;
+    float d = valid ? (vx * c.ax + vy * c.ay + vz * c.az) : 1.f;
+    c.cos_half = block_min(d, scratch);
+    return c;
+}
+
+// True when some pair (n in the normal cone, r in the direction cone) can
+// have n.r < 0, i.e. the per-pair shadow test (Propagation.h:1719) can
+// change the result.  Conservative: a 1e-3 rad margin covers the FP32
+// rounding of both cones.
+__device__ __forceinline__ bool cones_may_shadow(float nax, float nay, float naz, float ncos,
+                                                 const Cone &t) {
+    if (ncos <= 0.f || t.cos_half <= 0.f) return true;
+    float ca = fminf(1.f, fmaxf(-1.f, nax * t.ax + nay * t.ay + naz * t.az));
+    float ang = acosf(ca) + acosf(fminf(1.f, ncos)) + acosf(fminf(1.f, t.cos_half));
+    return !(ang < 1.5697963f);   // pi/2 - 1e-3
+}
+
+// Output bundles of the finish kernels (pointer lists in struct order).
+struct OutC2 { double *p[12]; };
+struct OutC4 { double *p[24]; };
+
+// Kernel timing with HIP events on the launch stream.
+struct Timer {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool on;
+    hipStream_t st;
+    Timer(bool enable, hipStream_t s) : on(enable), st(s) {
+        if (on) {
+            if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) on = false;
+            else (void)hipEventRecord(e0, st);
+        }
+    }
+    void stop() {
+        if (!on) return;
+        (void)hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) == hipSuccess) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            record_kernel_ms(ms);
+        }
+    }
+    ~Timer() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
+// Byte offset of the TileMeta array behind n_src packed records.
+__host__ __device__ __forceinline__ size_t meta_offset(int rec_bytes, int n_src) {
+    return ((size_t)rec_bytes * n_src + 255) / 256 * 256;
+}
+
+__host__ __device__ __forceinline__ int n_tiles(int n_src) { return (n_src + TILE - 1) / TILE; }
+
+// Launchers of the individual sum kernels (po_ff.hip, po_near.hip, po_scalar.hip).
+int launch_ff_sum(int precision, const SumPlan &p, const void *packed, int n_src, const GridPtrs &ct,
+                  int n_tgt, double rev_scale, double *partial, hipStream_t st);
+int launch_ff_finish(const SumPlan &p, const GridPtrs &ct, int n_tgt, double tdir, double pref,
+                     const c2Bundle &out, const double *partial, hipStream_t st);
+int launch_near_sum(int precision, const SumPlan &p, const void *packed, int n_src,
+                    const GridPtrs &ct, int n_tgt, double k, double tdir, double *partial,
+                    hipStream_t st);
+int launch_near_finish(int mode, const SumPlan &p, const GridPtrs &ct, int n_tgt, double pref,
+                       double zeta_inv, const OutC4 &out, const double *partial, hipStream_t st);
+int launch_scalar_sum(int precision, const SumPlan &p, const void *packed, int n_src,
+                      const GridPtrs &ct, int n_tgt, double rev_scale, double *partial,
+                      hipStream_t st);
+int launch_scalar_finish(const SumPlan &p, int n_tgt, double scale, double *re, double *im,
+                         const double *partial, hipStream_t st);
+
+}  // namespace pypo

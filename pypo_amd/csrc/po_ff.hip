@@ -1,0 +1,343 @@
+// po_ff.hip -- far-field PO sum (farfieldAtPoint, Propagation.h:1656-1781).
+//
+// Factored per target direction r:
+//   E = -j pref [SJ - (SJ.r) r + t r x SM],  H = -j pref [SM - (SM.r) r - t r x SJ]
+//   SJ = sum_s w_s A_s J_s exp(-j t k r_s.r),  SM likewise with M,
+// over the sources with n_s.r >= 0.  The sum kernels accumulate only SJ
+// and SM -- six complex multiply-adds per pair -- and the finish kernel
+// applies the projection, the cross product and -j pref in FP64.
+//
+// Per-pair work (mixed precision): the phase in revolutions
+// rev = x ux + y uy + z uz (FP64, u = r * (-t k / 2 pi)), its fraction
+// (v_fract_f64), FP32 sin/cos of the fraction, then FMAs.  Per tile (256
+// sources) two uniform decisions pick the inner loop: the shadow test is
+// skipped when the tile's normal cone and the workgroup's direction cone
+// prove n.r > 0 for every pair, and the M terms are skipped when the tile
+// carries no magnetic current (always so for PEC currents from the JM mode).
+// The per-tile FP32 partial sums are folded into FP64 accumulators.
+
+#include "po_common.hpp"
+
+namespace pypo {
+namespace {
+
+template <int TPL, bool HAS_M, bool MASK>
+__device__ __forceinline__ void ff_tile_mixed(const SrcRec32 *tile, const double (&rx)[TPL],
+                                              const double (&ry)[TPL], const double (&rz)[TPL],
+                                              const float (&fx)[TPL], const float (&fy)[TPL],
+                                              const float (&fz)[TPL], float (&acc)[TPL][12]) {
+#pragma unroll 2
+    for (int s = 0; s < TILE; s++) {
+        const SrcRec32 &r = tile[s];
+        const double x = r.x, y = r.y, z = r.z;
+        float cj[6], cm[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) cj[q] = r.cj[q];
+        if (HAS_M) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) cm[q] = r.cm[q];
+        }
+#pragma unroll
+        for (int j = 0; j < TPL; j++) {
+            double rev = __builtin_fma(z, rz[j], __builtin_fma(y, ry[j], x * rx[j]));
+            float f = (float)__builtin_amdgcn_fract(rev);
+            float sn, cs;
+            sincos_rev_f32(f, sn, cs);
+            if (MASK) {
+                float m = __builtin_fmaf(r.nz, fz[j], __builtin_fmaf(r.ny, fy[j], r.nx * fx[j]));
+                cs = (m < 0.f) ? 0.f : cs;
+                sn = (m < 0.f) ? 0.f : sn;
+            }
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                float &ar = acc[j][2 * q], &ai = acc[j][2 * q + 1];
+                ar = __builtin_fmaf(cj[2 * q], cs, ar);
+                ar = __builtin_fmaf(-cj[2 * q + 1], sn, ar);
+                ai = __builtin_fmaf(cj[2 * q], sn, ai);
+                ai = __builtin_fmaf(cj[2 * q + 1], cs, ai);
+            }
+            if (HAS_M) {
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    float &ar = acc[j][6 + 2 * q], &ai = acc[j][7 + 2 * q];
+                    ar = __builtin_fmaf(cm[2 * q], cs, ar);
+                    ar = __builtin_fmaf(-cm[2 * q + 1], sn, ar);
+                    ai = __builtin_fmaf(cm[2 * q], sn, ai);
+                    ai = __builtin_fmaf(cm[2 * q + 1], cs, ai);
+                }
+            }
+        }
+    }
+}
+
+template <int TPL>
+__global__ __launch_bounds__(256) void k_ff_mixed(const SrcRec32 *__restrict__ src,
+                                                  const TileMeta *__restrict__ meta, int n_src,
+                                                  int chunk, const double *__restrict__ th,
+                                                  const double *__restrict__ ph, int n_tgt,
+                                                  double rev_scale, double *__restrict__ partial) {
+    __shared__ SrcRec32 tile[TILE];
+    __shared__ float scratch[4];
+    const int tid = threadIdx.x;
+    double rx[TPL], ry[TPL], rz[TPL];
+    float fx[TPL], fy[TPL], fz[TPL];
+    Cone cone;
+    {
+        float cx = 0.f, cy = 0.f, cz = 0.f;
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < TPL; j++) {
+            int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+            double x = 0, y = 0, z = 0;
+            if (t < n_tgt) {
+                double st, ct, sp, cp;
+                sincos(th[t], &st, &ct);
+                sincos(ph[t], &sp, &cp);
+                x = cp * st; y = sp * st; z = ct;
+            }
+            fx[j] = (float)x; fy[j] = (float)y; fz[j] = (float)z;
+            rx[j] = x * rev_scale; ry[j] = y * rev_scale; rz[j] = z * rev_scale;
+            if (j == 0) { cx = fx[0]; cy = fy[0]; cz = fz[0]; any = t < n_tgt; }
+        }
+        // direction cone of the workgroup (first target of each lane, then
+        // widened by the other targets' deviation)
+        cone = block_cone(cx, cy, cz, any, scratch);
+        if (TPL > 1 && cone.cos_half > 0.f) {
+            float d = 1.f;
+#pragma unroll
+            for (int j = 1; j < TPL; j++) {
+                int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+                if (t < n_tgt) d = fminf(d, fx[j] * cone.ax + fy[j] * cone.ay + fz[j] * cone.az);
+            }
+            cone.cos_half = fminf(cone.cos_half, block_min(d, scratch));
+        }
+    }
+    float acc[TPL][12];
+    double tot[TPL][12];
+#pragma unroll
+    for (int j = 0; j < TPL; j++)
+#pragma unroll
+        for (int q = 0; q < 12; q++) { acc[j][q] = 0.f; tot[j][q] = 0.0; }
+
+    const int s_begin = blockIdx.y * chunk;
+    const int s_end = min(n_src, s_begin + chunk);
+    const uint4 *g = reinterpret_cast<const uint4 *>(src);
+    TileLoader<6> ld;
+    ld.load(g, s_begin + tid, s_end);
+    for (int base = s_begin; base < s_end; base += TILE) {
+        __syncthreads();
+        ld.store(reinterpret_cast<uint4 *>(tile), tid);
+        __syncthreads();
+        if (base + TILE < s_end) ld.load(g, base + TILE + tid, s_end);
+        const TileMeta m = meta[base / TILE];
+        const bool mask = cones_may_shadow(m.ax, m.ay, m.az, m.cos_half, cone);
+        if (m.has_m) {
+            if (mask) ff_tile_mixed<TPL, true, true>(tile, rx, ry, rz, fx, fy, fz, acc);
+            else ff_tile_mixed<TPL, true, false>(tile, rx, ry, rz, fx, fy, fz, acc);
+        } else {
+            if (mask) ff_tile_mixed<TPL, false, true>(tile, rx, ry, rz, fx, fy, fz, acc);
+            else ff_tile_mixed<TPL, false, false>(tile, rx, ry, rz, fx, fy, fz, acc);
+        }
+#pragma unroll
+        for (int j = 0; j < TPL; j++)
+#pragma unroll
+            for (int q = 0; q < 12; q++) { tot[j][q] += (double)acc[j][q]; acc[j][q] = 0.f; }
+    }
+#pragma unroll
+    for (int j = 0; j < TPL; j++) {
+        int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+        if (t < n_tgt) {
+#pragma unroll
+            for (int q = 0; q < 12; q++)
+                partial[((size_t)blockIdx.y * 12 + q) * n_tgt + t] = tot[j][q];
+        }
+    }
+}
+
+template <int TPL, bool HAS_M, bool MASK>
+__device__ __forceinline__ void ff_tile_fp64(const SrcRec64 *tile, const double (&rx)[TPL],
+                                             const double (&ry)[TPL], const double (&rz)[TPL],
+                                             const double (&ux)[TPL], const double (&uy)[TPL],
+                                             const double (&uz)[TPL], double (&acc)[TPL][12]) {
+#pragma unroll 1
+    for (int s = 0; s < TILE; s++) {
+        const SrcRec64 &r = tile[s];
+#pragma unroll
+        for (int j = 0; j < TPL; j++) {
+            double rev = __builtin_fma(r.z, rz[j], __builtin_fma(r.y, ry[j], r.x * rx[j]));
+            double f = __builtin_amdgcn_fract(rev);
+            double sn, cs;
+            sincospi(2.0 * f, &sn, &cs);
+            if (MASK) {
+                double m = __builtin_fma(r.nz, uz[j], __builtin_fma(r.ny, uy[j], r.nx * ux[j]));
+                cs = (m < 0.0) ? 0.0 : cs;
+                sn = (m < 0.0) ? 0.0 : sn;
+            }
+#pragma unroll
+            for (int q = 0; q < (HAS_M ? 6 : 3); q++) {
+                double &ar = acc[j][2 * q], &ai = acc[j][2 * q + 1];
+                ar = __builtin_fma(r.c[2 * q], cs, ar);
+                ar = __builtin_fma(-r.c[2 * q + 1], sn, ar);
+                ai = __builtin_fma(r.c[2 * q], sn, ai);
+                ai = __builtin_fma(r.c[2 * q + 1], cs, ai);
+            }
+        }
+    }
+}
+
+template <int TPL>
+__global__ __launch_bounds__(256) void k_ff_fp64(const SrcRec64 *__restrict__ src,
+                                                 const TileMeta *__restrict__ meta, int n_src,
+                                                 int chunk, const double *__restrict__ th,
+                                                 const double *__restrict__ ph, int n_tgt,
+                                                 double rev_scale, double *__restrict__ partial) {
+    __shared__ SrcRec64 tile[TILE];
+    __shared__ float scratch[4];
+    const int tid = threadIdx.x;
+    double ux[TPL], uy[TPL], uz[TPL];
+    double rx[TPL], ry[TPL], rz[TPL];
+    Cone cone;
+    {
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < TPL; j++) {
+            int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+            double x = 0, y = 0, z = 0;
+            if (t < n_tgt) {
+                double st, ct, sp, cp;
+                sincos(th[t], &st, &ct);
+                sincos(ph[t], &sp, &cp);
+                x = cp * st; y = sp * st; z = ct;
+            }
+            if (j == 0) any = t < n_tgt;
+            ux[j] = x; uy[j] = y; uz[j] = z;
+            rx[j] = x * rev_scale; ry[j] = y * rev_scale; rz[j] = z * rev_scale;
+        }
+        cone = block_cone((float)ux[0], (float)uy[0], (float)uz[0], any, scratch);
+        if (TPL > 1 && cone.cos_half > 0.f) {
+            float d = 1.f;
+#pragma unroll
+            for (int j = 1; j < TPL; j++) {
+                int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+                if (t < n_tgt) d = fminf(d, (float)(ux[j] * cone.ax + uy[j] * cone.ay + uz[j] * cone.az));
+            }
+            cone.cos_half = fminf(cone.cos_half, block_min(d, scratch));
+        }
+    }
+    double acc[TPL][12];
+#pragma unroll
+    for (int j = 0; j < TPL; j++)
+#pragma unroll
+        for (int q = 0; q < 12; q++) acc[j][q] = 0.0;
+
+    const int s_begin = blockIdx.y * chunk;
+    const int s_end = min(n_src, s_begin + chunk);
+    const uint4 *g = reinterpret_cast<const uint4 *>(src);
+    TileLoader<9> ld;
+    ld.load(g, s_begin + tid, s_end);
+    for (int base = s_begin; base < s_end; base += TILE) {
+        __syncthreads();
+        ld.store(reinterpret_cast<uint4 *>(tile), tid);
+        __syncthreads();
+        if (base + TILE < s_end) ld.load(g, base + TILE + tid, s_end);
+        const TileMeta m = meta[base / TILE];
+        const bool mask = cones_may_shadow(m.ax, m.ay, m.az, m.cos_half, cone);
+        if (m.has_m) {
+            if (mask) ff_tile_fp64<TPL, true, true>(tile, rx, ry, rz, ux, uy, uz, acc);
+            else ff_tile_fp64<TPL, true, false>(tile, rx, ry, rz, ux, uy, uz, acc);
+        } else {
+            if (mask) ff_tile_fp64<TPL, false, true>(tile, rx, ry, rz, ux, uy, uz, acc);
+            else ff_tile_fp64<TPL, false, false>(tile, rx, ry, rz, ux, uy, uz, acc);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < TPL; j++) {
+        int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+        if (t < n_tgt) {
+#pragma unroll
+            for (int q = 0; q < 12; q++)
+                partial[((size_t)blockIdx.y * 12 + q) * n_tgt + t] = acc[j][q];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ff_finish(const double *__restrict__ partial, int splits,
+                                                   int n_tgt, const double *__restrict__ th,
+                                                   const double *__restrict__ ph, double tdir,
+                                                   double pref, OutC2 out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tgt) return;
+    double S[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++) {
+        double v = 0.0;
+        for (int s = 0; s < splits; s++) v += partial[((size_t)s * 12 + q) * n_tgt + t];
+        S[q] = v;
+    }
+    double st, ct, sp, cp;
+    sincos(th[t], &st, &ct);
+    sincos(ph[t], &sp, &cp);
+    const double r[3] = {cp * st, sp * st, ct};
+    // complex components: J_a = (S[2a], S[2a+1]), M_a = (S[6+2a], S[7+2a])
+    double jdr_re = 0, jdr_im = 0, mdr_re = 0, mdr_im = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        jdr_re += S[2 * a] * r[a]; jdr_im += S[2 * a + 1] * r[a];
+        mdr_re += S[6 + 2 * a] * r[a]; mdr_im += S[7 + 2 * a] * r[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int b = (a + 1) % 3, c = (a + 2) % 3;
+        // (r x V)_a = r_b V_c - r_c V_b
+        double rxm_re = r[b] * S[6 + 2 * c] - r[c] * S[6 + 2 * b];
+        double rxm_im = r[b] * S[7 + 2 * c] - r[c] * S[7 + 2 * b];
+        double rxj_re = r[b] * S[2 * c] - r[c] * S[2 * b];
+        double rxj_im = r[b] * S[2 * c + 1] - r[c] * S[2 * b + 1];
+        double e_re = S[2 * a] - r[a] * jdr_re + tdir * rxm_re;
+        double e_im = S[2 * a + 1] - r[a] * jdr_im + tdir * rxm_im;
+        double h_re = S[6 + 2 * a] - r[a] * mdr_re - tdir * rxj_re;
+        double h_im = S[7 + 2 * a] - r[a] * mdr_im - tdir * rxj_im;
+        // multiply by -j*pref: (x + j y)(-j) = y - j x
+        out.p[a][t] = pref * e_im;
+        out.p[3 + a][t] = -pref * e_re;
+        out.p[6 + a][t] = pref * h_im;
+        out.p[9 + a][t] = -pref * h_re;
+    }
+}
+
+}  // namespace
+
+int launch_ff_sum(int precision, const SumPlan &p, const void *packed, int n_src, const GridPtrs &ct,
+                  int n_tgt, double rev_scale, double *partial, hipStream_t st) {
+    dim3 grid(p.tblocks, p.splits), block(TILE);
+    const int rb = precision == PYPO_PREC_FP64 ? (int)sizeof(SrcRec64) : (int)sizeof(SrcRec32);
+    const TileMeta *meta = (const TileMeta *)((const char *)packed + meta_offset(rb, n_src));
+    if (precision == PYPO_PREC_FP64) {
+        if (p.tpl == 2)
+            hipLaunchKernelGGL(k_ff_fp64<2>, grid, block, 0, st, (const SrcRec64 *)packed, meta, n_src,
+                               p.chunk, ct.x, ct.y, n_tgt, rev_scale, partial);
+        else
+            hipLaunchKernelGGL(k_ff_fp64<1>, grid, block, 0, st, (const SrcRec64 *)packed, meta, n_src,
+                               p.chunk, ct.x, ct.y, n_tgt, rev_scale, partial);
+    } else {
+        if (p.tpl == 2)
+            hipLaunchKernelGGL(k_ff_mixed<2>, grid, block, 0, st, (const SrcRec32 *)packed, meta, n_src,
+                               p.chunk, ct.x, ct.y, n_tgt, rev_scale, partial);
+        else
+            hipLaunchKernelGGL(k_ff_mixed<1>, grid, block, 0, st, (const SrcRec32 *)packed, meta, n_src,
+                               p.chunk, ct.x, ct.y, n_tgt, rev_scale, partial);
+    }
+    PYPO_CHECK_LAUNCH();
+    return 0;
+}
+
+int launch_ff_finish(const SumPlan &p, const GridPtrs &ct, int n_tgt, double tdir, double pref,
+                     const c2Bundle &b, const double *partial, hipStream_t st) {
+    OutC2 o{{b.r1x, b.r1y, b.r1z, b.i1x, b.i1y, b.i1z, b.r2x, b.r2y, b.r2z, b.i2x, b.i2y, b.i2z}};
+    hipLaunchKernelGGL(k_ff_finish, dim3((n_tgt + 255) / 256), dim3(256), 0, st, partial, p.splits,
+                       n_tgt, ct.x, ct.y, tdir, pref, o);
+    PYPO_CHECK_LAUNCH();
+    return 0;
+}
+
+}  // namespace pypo

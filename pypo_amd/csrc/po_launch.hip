@@ -1,0 +1,72 @@
+// po_launch.hip -- launch planning and the mode dispatch of one PO
+// propagation: packed sources -> sum kernel (grid = target blocks x source
+// splits) -> finish kernel (deterministic split reduction + epilogue).
+
+#include "po_common.hpp"
+
+namespace pypo {
+
+SumPlan plan_sum(int mode, int precision, int n_src, int n_tgt) {
+    SumPlan p;
+    // Two targets per lane halve the LDS reads per pair in the far field;
+    // the near-field and scalar kernels are register-heavy and keep one.
+    p.tpl = (mode == PYPO_MODE_FF && n_tgt >= 64 * 1024) ? 2 : 1;
+    (void)precision;
+    p.tblocks = (n_tgt + TILE * p.tpl - 1) / (TILE * p.tpl);
+    // Enough workgroups to fill 256 CUs several times over; never split a
+    // source range below 8 tiles.
+    const int want_wgs = 2048;
+    int splits = (want_wgs + p.tblocks - 1) / p.tblocks;
+    int max_splits = (n_src + 8 * TILE - 1) / (8 * TILE);
+    if (splits > max_splits) splits = max_splits;
+    if (splits < 1) splits = 1;
+    int chunk = (n_src + splits - 1) / splits;
+    chunk = (chunk + TILE - 1) / TILE * TILE;
+    p.chunk = chunk;
+    p.splits = (n_src + chunk - 1) / chunk;
+    if (p.splits < 1) p.splits = 1;
+    return p;
+}
+
+size_t partial_bytes(int mode, const SumPlan &p, int n_tgt) {
+    int comps = (mode == PYPO_MODE_SCALAR) ? 2 : 12;
+    return (size_t)p.splits * comps * (size_t)n_tgt * sizeof(double);
+}
+
+int launch_po(int mode, int precision, const void *packed, int n_src, const GridPtrs &ct,
+              int n_tgt, double k, double epsilon, double t_dir, void *res, double *partial_ws,
+              size_t ws_bytes, hipStream_t st, bool time_it) {
+    PYPO_REQUIRE(n_src > 0 && n_tgt > 0, ERR_ARG, "empty source or target grid");
+    PYPO_REQUIRE(mode >= PYPO_MODE_JM && mode <= PYPO_MODE_SCALAR, ERR_ARG, "unknown PO mode");
+    const SumPlan p = plan_sum(mode, precision, n_src, n_tgt);
+    PYPO_REQUIRE(partial_bytes(mode, p, n_tgt) <= ws_bytes, ERR_ARG, "partial workspace too small");
+    PYPO_REQUIRE((long long)p.chunk * p.splits >= n_src && p.chunk % TILE == 0, ERR_ARG,
+                 "internal: bad source split plan");
+    const PhysConsts pc = phys_consts(k, epsilon);
+    int err;
+
+    Timer tm(time_it, st);
+    if (mode == PYPO_MODE_FF)
+        err = launch_ff_sum(precision, p, packed, n_src, ct, n_tgt, -t_dir * k / kTwoPi, partial_ws, st);
+    else if (mode == PYPO_MODE_SCALAR)
+        err = launch_scalar_sum(precision, p, packed, n_src, ct, n_tgt, -t_dir * k / kTwoPi,
+                                partial_ws, st);
+    else
+        err = launch_near_sum(precision, p, packed, n_src, ct, n_tgt, k, t_dir, partial_ws, st);
+    if (err) return err;
+    tm.stop();
+
+    if (mode == PYPO_MODE_FF)
+        return launch_ff_finish(p, ct, n_tgt, t_dir, pc.prefactor, *(const c2Bundle *)res, partial_ws, st);
+    if (mode == PYPO_MODE_SCALAR) {
+        const arrC1 *b = (const arrC1 *)res;
+        return launch_scalar_finish(p, n_tgt, -k * k / (4 * pc.pi_f), b->x, b->y, partial_ws, st);
+    }
+    OutC4 o{};
+    const int nres = mode == PYPO_MODE_JMEH ? 24 : (mode == PYPO_MODE_EHP ? 15 : 12);
+    double *const *ptrs = (double *const *)res;   // every bundle is a sequence of double*
+    for (int q = 0; q < nres; q++) o.p[q] = ptrs[q];
+    return launch_near_finish(mode, p, ct, n_tgt, pc.prefactor, pc.zeta_inv, o, partial_ws, st);
+}
+
+}  // namespace pypo
