@@ -103,6 +103,17 @@ def feed_currents(w0):
     return T.currents(-amp + 0j, z, z, z, amp + 0j, z)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (profiles/pmc_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            return float(json.load(f)[kernel]["bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def rel_err(a, b):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
@@ -168,6 +179,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-rt", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="one launch of each kernel plus a 512 MiB device copy (counter calibration), "
+                         "for rocprofv3 runs; prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the PO sums shard over targets with no exchange, "
@@ -196,6 +210,17 @@ def main():
     def step(prec):
         D.propagate("JM", g_feed, g_dish, c_feed, feed, K, precision=prec, out=cur_dish)
         D.propagate("FF", g_dish, g_ff, cur_dish, dish, K, precision=prec, out=out_ff)
+
+    if args.profile:
+        step("mixed")
+        step("fp64")
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = src.clone()            # calibration copy: 512 MiB read + 512 MiB written
+        torch.cuda.synchronize()
+        if not args.no_rt:
+            run_rt(torch, D)
+        log(f"profile run done ({dst.numel() * 8 / 2**20:.0f} MiB calibration copy)")
+        return
 
     for _ in range(args.warmup):
         step("mixed")
@@ -287,7 +312,8 @@ def main():
                    "pairs_per_step": pairs_jm + pairs_ff, "jm_pairs": pairs_jm, "ff_pairs": pairs_ff,
                    "feed_waist_mm": round(w0, 5), "precision": "mixed"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": round(peak, 1),
-                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                     "traffic": measured_traffic("k_ff_mixed"),
                      "kernel": "k_ff_mixed", "kernel_ms": round(ff_ms, 3),
                      "work_per_pair": f"{FF_FLOPS_F32} FP32 + {FF_FLOPS_F64} FP64 flops + sincos"},
         "cpu_baseline": cpu,

@@ -23,8 +23,9 @@ constexpr int MAX_SURF = 4;
 struct Surf {
     double M[16];
     double a, b, c;
+    double ia2, ib2, ic2;   // 1/a^2, 1/b^2, 1/c^2 (computed once on the host)
     int type, flip;
-    const double *errs;   // device, or nullptr when rms == 0
+    const double *errs;     // device, or nullptr when rms == 0
 };
 
 struct SurfSet {
@@ -40,29 +41,23 @@ struct FrameSet {
     FramePtrs f[MAX_SURF + 1];
 };
 
-__device__ __forceinline__ double q1(double t, double x, double y, double dx, double dy, double a,
-                                     double b) {
-    return (x + t * dx) * (x + t * dx) / (a * a) + (y + t * dy) * (y + t * dy) / (b * b);
-}
-
-__device__ __forceinline__ double q2(double t, double x, double y, double dx, double dy, double a,
-                                     double b) {
-    return (x + t * dx) * 2 * dx / (a * a) + (y + t * dy) * 2 * dy / (b * b);
-}
-
-// One Newton update for the quadric (RTRefls.h gp 96, gh 120, ge 145, gpl 170).
-__device__ __forceinline__ double newton(int type, double t, double x, double y, double z, double dx,
-                                         double dy, double dz, double a, double b, double c) {
-    if (type == 0) return t - (z + t * dz - q1(t, x, y, dx, dy, a, b)) / (dz - q2(t, x, y, dx, dy, a, b));
-    if (type == 1) {
-        double s = sqrt(q1(t, x, y, dx, dy, a, b) + 1);
-        return t - (z + t * dz - c * s) / (dz - c / (2 * s) * q2(t, x, y, dx, dy, a, b));
+// One Newton update t <- t - f(t)/f'(t) for the quadric (RTRefls.h gp 96,
+// gh 120, ge 145, gpl 170).  common1/common2 (RTRefls.h:52-94) are
+// evaluated once per step with the reciprocal squared semi-axes; this moves
+// intermediate values by at most an ulp against the reference's x/(a*a).
+__device__ __forceinline__ double newton(const Surf &s, double t, double x, double y, double z,
+                                         double dx, double dy, double dz) {
+    const double X = x + t * dx, Y = y + t * dy;
+    if (s.type == 3) return t - (z + t * dz) / dz;
+    const double c1 = X * X * s.ia2 + Y * Y * s.ib2;
+    const double c2 = 2.0 * (X * dx * s.ia2 + Y * dy * s.ib2);
+    if (s.type == 0) return t - (z + t * dz - c1) / (dz - c2);
+    if (s.type == 1) {
+        const double r = sqrt(c1 + 1);
+        return t - (z + t * dz - s.c * r) / (dz - s.c / (2 * r) * c2);
     }
-    if (type == 2) {
-        double s = sqrt(1 - q1(t, x, y, dx, dy, a, b));
-        return t - (z + t * dz - c * s) / (dz + c / (2 * s) * q2(t, x, y, dx, dy, a, b));
-    }
-    return t - (z + t * dz) / dz;
+    const double r = sqrt(1 - c1);
+    return t - (z + t * dz - s.c * r) / (dz + s.c / (2 * r) * c2);
 }
 
 __global__ __launch_bounds__(256) void k_trace(SurfSet S, FrameSet F, int n, double tol, double t0,
@@ -91,7 +86,7 @@ __global__ __launch_bounds__(256) void k_trace(SurfSet S, FrameSet F, int n, dou
         double chk = fabs(t1 - t);
         int it = 0;
         while (chk > tol && it < max_iter) {
-            t1 = newton(s.type, t, x, y, z, dx, dy, dz, s.a, s.b, s.c);
+            t1 = newton(s, t, x, y, z, dx, dy, dz);
             chk = fabs(t1 - t);
             t = t1;
             ++it;
@@ -111,13 +106,13 @@ __global__ __launch_bounds__(256) void k_trace(SurfSet S, FrameSet F, int n, dou
             } else {
                 double fl = (double)s.flip;
                 if (s.type == 0) {
-                    nx = -2 * px / (s.a * s.a) * fl; ny = -2 * py / (s.b * s.b) * fl; nz = fl;
+                    nx = -2 * px * s.ia2 * fl; ny = -2 * py * s.ib2 * fl; nz = fl;
                 } else if (s.type == 1) {
-                    nx = -2 * px / (s.a * s.a) * fl; ny = -2 * py / (s.b * s.b) * fl;
-                    nz = 2 * pz / (s.c * s.c) * fl;
+                    nx = -2 * px * s.ia2 * fl; ny = -2 * py * s.ib2 * fl;
+                    nz = 2 * pz * s.ic2 * fl;
                 } else {
-                    nx = 2 * px / (s.a * s.a) * fl; ny = 2 * py / (s.b * s.b) * fl;
-                    nz = 2 * pz / (s.c * s.c) * fl;
+                    nx = 2 * px * s.ia2 * fl; ny = 2 * py * s.ib2 * fl;
+                    nz = 2 * pz * s.ic2 * fl;
                 }
                 double nrm = sqrt(nx * nx + ny * ny + nz * nz);
                 nx /= nrm; ny /= nrm; nz /= nrm;
@@ -159,6 +154,9 @@ int launch_trace_chain(const reflparams *surfs, const double *const *d_errs, int
         PYPO_REQUIRE(r.type >= 0 && r.type <= 3, ERR_ARG, "unknown surface type");
         for (int q = 0; q < 16; q++) S.s[k].M[q] = r.transf[q];
         S.s[k].a = r.coeffs[0]; S.s[k].b = r.coeffs[1]; S.s[k].c = r.coeffs[2];
+        S.s[k].ia2 = 1.0 / (r.coeffs[0] * r.coeffs[0]);
+        S.s[k].ib2 = 1.0 / (r.coeffs[1] * r.coeffs[1]);
+        S.s[k].ic2 = 1.0 / (r.coeffs[2] * r.coeffs[2]);
         S.s[k].type = r.type;
         S.s[k].flip = r.flip ? -1 : 1;
         S.s[k].errs = d_errs ? d_errs[k] : nullptr;
