@@ -192,6 +192,44 @@ int pypo_hip_generate_grid_dev(reflparams refl, reflcontainer *container, void *
 int pypo_hip_trace_dev(const reflparams *surfs, int n_surf, cframe *frames,
                        double tol, double t0, void *stream);
 
+/* ---- batched sweeps ---------------------------------------------------- */
+
+/* A sweep of n_inst JM -> FF instances (e.g. frequencies x feed offsets) of
+ * uneven grid sizes sharing one far-field grid.  Per instance: feed surface,
+ * dish surface, wavenumber k (1/mm) and the feed currents (host arrays of the
+ * feed grid's size).  The JM step (feed -> dish, PEC currents) and the FF
+ * step (dish -> ff) of every instance run as one persistent launch each over
+ * a static work list sorted by cost; mixed precision.  tpl (targets per lane,
+ * 1 or 2), chunk_tiles (256-source tiles per work item) and wgs_per_cu
+ * (persistent workgroups per CU) shape the launch; 0 picks defaults.
+ * use_graph captures the whole run into a hipGraph that pypo_hip_sweep_run
+ * replays.  No counterpart in the reference (its batch loops are Python). */
+typedef struct pypo_sweep_desc {
+    int n_inst;
+    const reflparams *feeds;
+    const reflparams *dishes;
+    const double *k;
+    const c2Bundle *feed_currents;
+    reflparams ff;
+    double t_direction;
+    int tpl, chunk_tiles, wgs_per_cu, use_graph;
+} pypo_sweep_desc;
+
+int pypo_hip_sweep_create(const pypo_sweep_desc *desc, void **handle);
+/* Rebuild the work lists (and graph) with new launch shapes. */
+int pypo_hip_sweep_plan(void *handle, int tpl, int chunk_tiles, int wgs_per_cu, int use_graph);
+/* Run (replay) the sweep on `stream` (NULL: default stream); asynchronous. */
+int pypo_hip_sweep_run(void *handle, void *stream);
+/* Copy instance inst's far field (12 arrays of the ff grid size) / dish
+ * currents (12 arrays of the dish size) to host; synchronous. */
+int pypo_hip_sweep_read(void *handle, int inst, c2Bundle *ff_out);
+int pypo_hip_sweep_read_currents(void *handle, int inst, c2Bundle *dish_currents);
+/* Pair counts, work-list lengths and the device pointer of the far-field
+ * results ([n_inst][12][n_ff] doubles, c2Bundle row order). */
+int pypo_hip_sweep_info(void *handle, double *pairs_jm, double *pairs_ff, int *items_jm,
+                        int *items_ff, double **ff_out_dev);
+int pypo_hip_sweep_destroy(void *handle);
+
 /* Mean kernel time (ms) of the last pypo_hip_propagate_dev / trace call's
  * dominant kernel, measured with HIP events on the launch stream, when
  * timing is enabled by pypo_hip_set_timing(1).  Returns -1 if unavailable. */

@@ -520,4 +520,58 @@ int pypo_hip_trace_dev(const reflparams *surfs, int n_surf, cframe *frames, doub
     return end_call(st);
 }
 
+// ---- batched sweeps ---------------------------------------------------------------
+
+int pypo_hip_sweep_create(const pypo_sweep_desc *desc, void **handle) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    PYPO_REQUIRE(handle, ERR_ARG, "sweep: NULL handle");
+    *handle = nullptr;
+    Sweep *s = nullptr;
+    if ((err = sweep_create(desc, &s, g_stream))) return err;
+    *handle = s;
+    return 0;
+}
+
+int pypo_hip_sweep_plan(void *handle, int tpl, int chunk_tiles, int wgs_per_cu, int use_graph) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    PYPO_REQUIRE(handle, ERR_ARG, "sweep: NULL handle");
+    SweepPlanParams prm{tpl, chunk_tiles, wgs_per_cu};
+    return sweep_plan((Sweep *)handle, prm, use_graph != 0, g_stream);
+}
+
+int pypo_hip_sweep_run(void *handle, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    PYPO_REQUIRE(handle, ERR_ARG, "sweep: NULL handle");
+    return sweep_run((Sweep *)handle, (hipStream_t)stream);
+}
+
+int pypo_hip_sweep_read(void *handle, int inst, c2Bundle *ff_out) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    PYPO_REQUIRE(handle && ff_out, ERR_ARG, "sweep: NULL argument");
+    PYPO_HIP_TRY(hipDeviceSynchronize());
+    return sweep_read((Sweep *)handle, inst, ff_out, g_stream);
+}
+
+int pypo_hip_sweep_read_currents(void *handle, int inst, c2Bundle *dish_currents) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    PYPO_REQUIRE(handle && dish_currents, ERR_ARG, "sweep: NULL argument");
+    PYPO_HIP_TRY(hipDeviceSynchronize());
+    return sweep_read_currents((Sweep *)handle, inst, dish_currents, g_stream);
+}
+
+int pypo_hip_sweep_info(void *handle, double *pairs_jm, double *pairs_ff, int *items_jm,
+                        int *items_ff, double **ff_out_dev) {
+    PYPO_REQUIRE(handle, ERR_ARG, "sweep: NULL handle");
+    sweep_info((const Sweep *)handle, pairs_jm, pairs_ff, items_jm, items_ff, ff_out_dev);
+    return 0;
+}
+
+int pypo_hip_sweep_destroy(void *handle) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (handle) sweep_destroy((Sweep *)handle);
+    return 0;
+}
+
 }  // extern "C"

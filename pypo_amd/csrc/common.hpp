@@ -208,6 +208,20 @@ int launch_trace_chain(const reflparams *surfs, const double *const *d_errs, int
                        const cframe *frames, double tol, double t0, int max_iter, hipStream_t st,
                        bool time_it);
 
+// Batched sweeps (sweep.hip).
+struct Sweep;
+int sweep_create(const pypo_sweep_desc *d, Sweep **out, hipStream_t st);
+int sweep_plan(Sweep *s, struct SweepPlanParams prm, bool use_graph, hipStream_t st);
+int sweep_run(Sweep *s, hipStream_t st);
+int sweep_read(const Sweep *s, int inst, c2Bundle *host, hipStream_t st);
+int sweep_read_currents(const Sweep *s, int inst, c2Bundle *host, hipStream_t st);
+void sweep_info(const Sweep *s, double *pairs_jm, double *pairs_ff, int *n_items_jm, int *n_items_ff,
+                double **ff_out);
+void sweep_destroy(Sweep *s);
+struct SweepPlanParams {
+    int tpl, chunk_tiles, wgs_per_cu;
+};
+
 // Kernel timing of the last timed launch (HIP events on the launch stream).
 void record_kernel_ms(double ms);
 

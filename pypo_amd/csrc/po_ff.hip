@@ -16,59 +16,10 @@
 // carries no magnetic current (always so for PEC currents from the JM mode).
 // The per-tile FP32 partial sums are folded into FP64 accumulators.
 
-#include "po_common.hpp"
+#include "po_tiles.hpp"
 
 namespace pypo {
 namespace {
-
-template <int TPL, bool HAS_M, bool MASK>
-__device__ __forceinline__ void ff_tile_mixed(const SrcRec32 *tile, const double (&rx)[TPL],
-                                              const double (&ry)[TPL], const double (&rz)[TPL],
-                                              const float (&fx)[TPL], const float (&fy)[TPL],
-                                              const float (&fz)[TPL], float (&acc)[TPL][12]) {
-#pragma unroll 2
-    for (int s = 0; s < TILE; s++) {
-        const SrcRec32 &r = tile[s];
-        const double x = r.x, y = r.y, z = r.z;
-        float cj[6], cm[6];
-#pragma unroll
-        for (int q = 0; q < 6; q++) cj[q] = r.cj[q];
-        if (HAS_M) {
-#pragma unroll
-            for (int q = 0; q < 6; q++) cm[q] = r.cm[q];
-        }
-#pragma unroll
-        for (int j = 0; j < TPL; j++) {
-            double rev = __builtin_fma(z, rz[j], __builtin_fma(y, ry[j], x * rx[j]));
-            float f = (float)__builtin_amdgcn_fract(rev);
-            float sn, cs;
-            sincos_rev_f32(f, sn, cs);
-            if (MASK) {
-                float m = __builtin_fmaf(r.nz, fz[j], __builtin_fmaf(r.ny, fy[j], r.nx * fx[j]));
-                cs = (m < 0.f) ? 0.f : cs;
-                sn = (m < 0.f) ? 0.f : sn;
-            }
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                float &ar = acc[j][2 * q], &ai = acc[j][2 * q + 1];
-                ar = __builtin_fmaf(cj[2 * q], cs, ar);
-                ar = __builtin_fmaf(-cj[2 * q + 1], sn, ar);
-                ai = __builtin_fmaf(cj[2 * q], sn, ai);
-                ai = __builtin_fmaf(cj[2 * q + 1], cs, ai);
-            }
-            if (HAS_M) {
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    float &ar = acc[j][6 + 2 * q], &ai = acc[j][7 + 2 * q];
-                    ar = __builtin_fmaf(cm[2 * q], cs, ar);
-                    ar = __builtin_fmaf(-cm[2 * q + 1], sn, ar);
-                    ai = __builtin_fmaf(cm[2 * q], sn, ai);
-                    ai = __builtin_fmaf(cm[2 * q + 1], cs, ai);
-                }
-            }
-        }
-    }
-}
 
 template <int TPL>
 __global__ __launch_bounds__(256) void k_ff_mixed(const SrcRec32 *__restrict__ src,
