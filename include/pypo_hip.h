@@ -230,6 +230,33 @@ int pypo_hip_sweep_info(void *handle, double *pairs_jm, double *pairs_ff, int *i
                         int *items_ff, double **ff_out_dev);
 int pypo_hip_sweep_destroy(void *handle);
 
+/* ---- adjoint of the far-field power (no counterpart in the reference) ----
+ * Surface-shape gradients for reflector optimisation.  The dish is
+ * perturbed as the reference perturbs it for surface errors
+ * (InterfaceReflector.cpp:852-866): points move along their normals,
+ * normals and area elements stay.  Forward: feed currents -> JM on the
+ * displaced dish -> FF; P_t = |E_t|^2.
+ *
+ * out = base with each point moved by d[i] along its normal (device
+ * arrays; base->size points). */
+int pypo_hip_displace_dev(const reflcontainer *base, const double *d, reflcontainer *out,
+                          void *stream);
+/* grad[i] = dL/dd[i] for a loss L of the far-field power with
+ * dl_dp[t] = dL/dP_t.  dish: the displaced dish grid (gmode/ncx/ncy its
+ * n_cells); ff_fields: the FF result, on the ff grid, of the JM currents
+ * that feed_currents induce on the dish, both computed with the same k and
+ * t_direction.  The dish currents and their derivatives are recomputed
+ * here from the feed (FP64), not read.  All arrays device memory;
+ * precision PYPO_PREC_MIXED or PYPO_PREC_FP64 selects the reverse
+ * far-field sum's arithmetic.  Asynchronous on `stream`, ordered like
+ * pypo_hip_propagate_dev. */
+int pypo_hip_ff_power_grad_dev(const reflcontainer *dish, int gmode, int ncx, int ncy,
+                               const reflcontainer *feed,
+                               int feed_gmode, int feed_ncx, int feed_ncy,
+                               const c2Bundle *feed_currents, const reflcontainer *ff,
+                               const c2Bundle *ff_fields, const double *dl_dp, double k,
+                               double t_direction, int precision, double *grad, void *stream);
+
 /* Mean kernel time (ms) of the last pypo_hip_propagate_dev / trace call's
  * dominant kernel, measured with HIP events on the launch stream, when
  * timing is enabled by pypo_hip_set_timing(1).  Returns -1 if unavailable. */

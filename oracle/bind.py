@@ -158,3 +158,49 @@ def farfield_subset(lib, source, target, currents, k, idx, nthreads=None, epsilo
     dt = time.perf_counter() - t
     o = MU.c2bundle_to_obj(res, (idx.size,), "fields")
     return np.stack(o.components()), dt
+
+
+def _container_from_rows(rows):
+    """reflcontainer over a (7, n) float64 array (rows x y z nx ny nz area)."""
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    c = PS.reflcontainer()
+    c.size = rows.shape[1]
+    for i, name in enumerate(("x", "y", "z", "nx", "ny", "nz", "area")):
+        setattr(c, name, rows[i].ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    c._keep = rows
+    return c
+
+
+def _bundle_from_rows(rows):
+    """c2Bundle over a (12, n) float64 array in c2Bundle row order."""
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    b = PS.c2Bundle()
+    for i, (name, _) in enumerate(b._fields_):
+        setattr(b, name, rows[i].ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    b._keep = rows
+    return b
+
+
+def propagate_on_grids(lib, mode, src_rows, cells, tgt_rows, cur_rows, k, nthreads=None, epsilon=1.0,
+                       exp="fwd"):
+    """JM / EH / FF from (7, n_s) source rows onto (7, n_t) target rows that
+    the caller generated (port only; no grid generation inside).  cells =
+    (gmode, ncx, ncy) of the source grid; cur_rows (12, n_s).  Returns the
+    (12, n_t) result rows in c2Bundle order."""
+    modes = {"JM": 0, "EH": 1, "FF": 4}
+    nthreads = nthreads or default_threads()
+    cs, ct = _container_from_rows(src_rows), _container_from_rows(tgt_rows)
+    cur = _bundle_from_rows(cur_rows)
+    out = np.zeros((12, ct.size))
+    res = _bundle_from_rows(out)
+    f = lib.cdll.oracle_propagate_on_grids
+    f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                  ctypes.POINTER(PS.reflcontainer), ctypes.POINTER(PS.reflcontainer),
+                  ctypes.POINTER(PS.c2Bundle), ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                  ctypes.c_double]
+    f.restype = None
+    gmode, ncx, ncy = cells
+    f(modes[mode], ctypes.cast(ctypes.byref(res), ctypes.c_void_p), int(gmode), int(ncx), int(ncy),
+      ctypes.byref(cs), ctypes.byref(ct), ctypes.byref(cur), float(k), int(nthreads), float(epsilon),
+      -1.0 if exp == "fwd" else 1.0)
+    return res._keep

@@ -428,13 +428,11 @@ enum Mode { JM = 0, EH = 1, JMEH = 2, EHP = 3, FF = 4 };
 
 // Per-target epilogues: propagateBeam_JM_PEC 269-323, _EH 574-617,
 // _JMEH_PEC 482-555, _EHP 805-914, propagateToFarField 1581-1639.
-void propagate(int mode, void *res, const reflparams &sp, const reflparams &tp,
-               reflcontainer *cs, reflcontainer *ct, const c2Bundle *cur,
-               double k, int nthreads, double epsilon, double t) {
-    make_grid(sp, cs, true, true);
-    make_grid(tp, ct, true, true);
+void propagate_on(int mode, void *res, int gmode, int ncx, int ncy, reflcontainer *cs,
+                  reflcontainer *ct, const c2Bundle *cur, double k, int nthreads, double epsilon,
+                  double t) {
     Consts K(k, epsilon);
-    Src S{cs, cur, sp.gmode, sp.n_cells[0], sp.n_cells[1]};
+    Src S{cs, cur, gmode, ncx, ncy};
     split_run(ct->size, nthreads, [&](int lo, int hi) {
         for (int i = lo; i < hi; i++) {
             cplx E[3], H[3];
@@ -510,6 +508,14 @@ void propagate(int mode, void *res, const reflparams &sp, const reflparams &tp,
             }
         }
     });
+}
+
+void propagate(int mode, void *res, const reflparams &sp, const reflparams &tp,
+               reflcontainer *cs, reflcontainer *ct, const c2Bundle *cur,
+               double k, int nthreads, double epsilon, double t) {
+    make_grid(sp, cs, true, true);
+    make_grid(tp, ct, true, true);
+    propagate_on(mode, res, sp.gmode, sp.n_cells[0], sp.n_cells[1], cs, ct, cur, k, nthreads, epsilon, t);
 }
 
 // ---------------------------------------------------------------------
@@ -675,6 +681,14 @@ void oracle_propagateToGrid_scalar(arrC1 *res, reflparams s, reflparams t, reflc
 void oracle_propagateRays(reflparams ctp, cframe *fr_in, cframe *fr_out, int nt, double tol,
                           double t0, int max_iter) {
     trace(ctp, fr_in, fr_out, nt, tol, t0, max_iter);
+}
+
+// A vector PO mode (JM, EH, JMEH, EHP, FF) on grids the caller generated
+// (and possibly displaced): no grid generation inside.
+void oracle_propagate_on_grids(int mode, void *res, int gmode, int ncx, int ncy, reflcontainer *cs,
+                               reflcontainer *ct, c2Bundle *cur, double k, int nt, double eps,
+                               double tdir) {
+    propagate_on(mode, res, gmode, ncx, ncy, cs, ct, cur, k, nt, eps, tdir);
 }
 
 // Far field on a strided subset of targets (bench CPU baseline): targets

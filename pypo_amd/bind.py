@@ -94,6 +94,12 @@ def loadHIPlib():
         lib.pypo_hip_generate_grid_dev.restype = i
         lib.pypo_hip_trace_dev.argtypes = [P(PS.reflparams), i, P(PS.cframe), d, d, ctypes.c_void_p]
         lib.pypo_hip_trace_dev.restype = i
+        RC, CB = P(PS.reflcontainer), P(PS.c2Bundle)
+        lib.pypo_hip_displace_dev.argtypes = [RC, P(d), RC, ctypes.c_void_p]
+        lib.pypo_hip_displace_dev.restype = i
+        lib.pypo_hip_ff_power_grad_dev.argtypes = [RC, i, i, i, RC, i, i, i, CB, RC, CB, P(d), d, d,
+                                                   i, P(d), ctypes.c_void_p]
+        lib.pypo_hip_ff_power_grad_dev.restype = i
         _lib = lib
         return lib
 

@@ -574,4 +574,53 @@ int pypo_hip_sweep_destroy(void *handle) {
     return 0;
 }
 
+// ---- adjoint of the far-field power -------------------------------------------------
+
+int pypo_hip_displace_dev(const reflcontainer *base, const double *d, reflcontainer *out,
+                          void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    PYPO_REQUIRE(d, ERR_ARG, "displace: NULL displacement");
+    if ((err = check_container(base, "base")) || (err = check_container(out, "out"))) return err;
+    PYPO_REQUIRE(base->size > 0 && out->size == base->size, ERR_ARG,
+                 "displace: base and out must have the same non-zero size");
+    hipStream_t st = (hipStream_t)stream;   // NULL: the default stream
+    if ((err = begin_call(st))) return err;
+    if ((err = launch_displace(grid_ptrs(*base), d, base->size, grid_ptrs(*out), st))) return err;
+    return end_call(st);
+}
+
+int pypo_hip_ff_power_grad_dev(const reflcontainer *dish, int gmode, int ncx, int ncy,
+                               const reflcontainer *feed,
+                               int feed_gmode, int feed_ncx, int feed_ncy,
+                               const c2Bundle *feed_currents, const reflcontainer *ff,
+                               const c2Bundle *ff_fields, const double *dl_dp, double k,
+                               double t_direction, int precision, double *grad, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    PYPO_REQUIRE(precision == PYPO_PREC_MIXED || precision == PYPO_PREC_FP64, ERR_ARG,
+                 "unknown precision");
+    PYPO_REQUIRE(feed_currents && ff_fields && dl_dp && grad, ERR_ARG,
+                 "adjoint: NULL argument");
+    if ((err = check_container(dish, "dish")) || (err = check_container(feed, "feed")) ||
+        (err = check_container(ff, "ff")))
+        return err;
+    PYPO_REQUIRE(k > 0 && (t_direction == 1.0 || t_direction == -1.0), ERR_ARG,
+                 "adjoint: k must be positive and t_direction +-1");
+    const size_t wsb = adjoint_workspace_bytes(precision, feed->size, dish->size, ff->size);
+    hipStream_t st = (hipStream_t)stream;   // NULL: the default stream
+    if ((err = begin_call(st))) return err;
+    void *ws = pool_get(SLOT_ADJ, wsb, &err);
+    if (err) return err;
+    if ((err = launch_ff_power_grad(precision, grid_ptrs(*dish), dish->size, gmode, ncx, ncy,
+                                    grid_ptrs(*feed), feed->size, feed_gmode,
+                                    feed_ncx, feed_ncy, bundle12(*feed_currents), grid_ptrs(*ff),
+                                    ff->size, bundle12(*ff_fields), dl_dp, k, t_direction, grad, ws,
+                                    wsb, st)))
+        return err;
+    return end_call(st);
+}
+
 }  // extern "C"

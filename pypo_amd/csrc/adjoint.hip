@@ -1,0 +1,417 @@
+// adjoint.hip -- gradient of a far-field power loss with respect to per-point
+// reflector displacement along the surface normal.
+//
+// Model (the reference's surface-error model, InterfaceReflector.cpp:852-866:
+// points move along their normals, normals and area elements stay):
+//   p_i = p0_i + d_i n_i
+//   J_i = 2 n_i x H_inc(p_i)                 (JM, Propagation.h:269-323)
+//   E_t = -j pref P_t sum_i c_i J_i exp(j phi_ti),  phi_ti = -t k p_i.r_t,
+//   c_i = w_i A_i                              (FF, Propagation.h:1656-1781)
+//   P_t = |E_t|^2 ;  g_t = dL/dP_t given by the caller.
+// Backward:
+//   A_t = 2 g_t (-j pref) conj(E_t)                                  (targets)
+//   B_i = sum_t A_t exp(j phi_ti),  G_i = sum_t (n_i.r_t) A_t exp(j phi_ti) (reverse sums)
+//   J_i, dJ_i = 2 n_i x (n_i . grad) H_inc(p_i)   (one pass over the feed: the
+//                                                   currents are recomputed, not stored)
+//   dL/dd_i = c_i [ Re(B_i . dJ_i) + t k Im(J_i . G_i) ]
+// The shadow tests of the forward sums are applied here too; their own
+// derivatives are zero almost everywhere.
+
+#include "po_common.hpp"
+
+#include <algorithm>
+
+namespace pypo {
+namespace {
+
+// Target record of the reverse far-field sum: r scaled to revolutions
+// (FP64), r (FP32) and A_t (3 complex, FP32).  64 bytes.
+struct __attribute__((aligned(16))) TgtRec32 {
+    double ux, uy, uz;
+    float rx, ry, rz;
+    float a[6];
+    float pad;
+};
+static_assert(sizeof(TgtRec32) == 64, "TgtRec32 must be 64 bytes");
+
+struct __attribute__((aligned(16))) TgtRec64 {
+    double ux, uy, uz;
+    double rx, ry, rz;
+    double a[6];
+};
+static_assert(sizeof(TgtRec64) == 96, "TgtRec64 must be 96 bytes");
+
+// One thread per far-field target: r_t, A_t from E_t and g_t.
+template <class Rec>
+__global__ __launch_bounds__(256) void k_adj_targets(const double *__restrict__ th,
+                                                     const double *__restrict__ ph, Bundle12 e,
+                                                     const double *__restrict__ g, int n_tgt,
+                                                     double rev_scale, double pref, Rec *__restrict__ out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tgt) return;
+    double st, ct, sp, cp;
+    sincos(th[t], &st, &ct);
+    sincos(ph[t], &sp, &cp);
+    const double r[3] = {cp * st, sp * st, ct};
+    Rec o{};
+    o.ux = r[0] * rev_scale; o.uy = r[1] * rev_scale; o.uz = r[2] * rev_scale;
+    o.rx = r[0]; o.ry = r[1]; o.rz = r[2];
+    const double s = 2.0 * g[t] * pref;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        // (-j)(Er - j Ei) = -Ei - j Er
+        o.a[2 * a] = s * (-e.p[3 + a][t]);
+        o.a[2 * a + 1] = s * (-e.p[a][t]);
+    }
+    out[t] = o;
+}
+
+// Reverse far-field sum, mixed precision: lanes own sources (dish points),
+// targets stream through LDS.  Partial layout [split][12][n_src]:
+// q 0..5 = B (re/im xyz interleaved), 6..11 = G.
+__global__ __launch_bounds__(256) void k_adj_reverse32(const TgtRec32 *__restrict__ tg, int n_tgt,
+                                                       int chunk, GridPtrs src, int n_src,
+                                                       double *__restrict__ partial) {
+    __shared__ TgtRec32 tile[TILE];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * TILE + tid;
+    const bool valid = i < n_src;
+    const double px = valid ? src.x[i] : 0.0, py = valid ? src.y[i] : 0.0, pz = valid ? src.z[i] : 0.0;
+    const float nx = valid ? (float)src.nx[i] : 0.f, ny = valid ? (float)src.ny[i] : 0.f,
+                nz = valid ? (float)src.nz[i] : 0.f;
+    float acc[12];
+    double tot[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++) { acc[q] = 0.f; tot[q] = 0.0; }
+    const int t_begin = blockIdx.y * chunk;
+    const int t_end = min(n_tgt, t_begin + chunk);
+    for (int base = t_begin; base < t_end; base += TILE) {
+        __syncthreads();
+        const int t = base + tid;
+        if (t < t_end) {
+            tile[tid] = tg[t];
+        } else {
+            TgtRec32 z{};
+            tile[tid] = z;   // zero A: adds nothing
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int s = 0; s < TILE; s++) {
+            const TgtRec32 &r = tile[s];
+            double rev = __builtin_fma(pz, r.uz, __builtin_fma(py, r.uy, px * r.ux));
+            float f = (float)__builtin_amdgcn_fract(rev);
+            float sn, cs;
+            sincos_rev_f32(f, sn, cs);
+            float w = __builtin_fmaf(nz, r.rz, __builtin_fmaf(ny, r.ry, nx * r.rx));
+            cs = (w < 0.f) ? 0.f : cs;
+            sn = (w < 0.f) ? 0.f : sn;
+            const float wc = w * cs, ws = w * sn;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const float ar = r.a[2 * a], ai = r.a[2 * a + 1];
+                acc[2 * a] = __builtin_fmaf(ar, cs, acc[2 * a]);
+                acc[2 * a] = __builtin_fmaf(-ai, sn, acc[2 * a]);
+                acc[2 * a + 1] = __builtin_fmaf(ar, sn, acc[2 * a + 1]);
+                acc[2 * a + 1] = __builtin_fmaf(ai, cs, acc[2 * a + 1]);
+                acc[6 + 2 * a] = __builtin_fmaf(ar, wc, acc[6 + 2 * a]);
+                acc[6 + 2 * a] = __builtin_fmaf(-ai, ws, acc[6 + 2 * a]);
+                acc[7 + 2 * a] = __builtin_fmaf(ar, ws, acc[7 + 2 * a]);
+                acc[7 + 2 * a] = __builtin_fmaf(ai, wc, acc[7 + 2 * a]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 12; q++) { tot[q] += (double)acc[q]; acc[q] = 0.f; }
+    }
+    if (valid) {
+#pragma unroll
+        for (int q = 0; q < 12; q++) partial[((size_t)blockIdx.y * 12 + q) * n_src + i] = tot[q];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_adj_reverse64(const TgtRec64 *__restrict__ tg, int n_tgt,
+                                                       int chunk, GridPtrs src, int n_src,
+                                                       double *__restrict__ partial) {
+    __shared__ TgtRec64 tile[TILE];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * TILE + tid;
+    const bool valid = i < n_src;
+    const double px = valid ? src.x[i] : 0.0, py = valid ? src.y[i] : 0.0, pz = valid ? src.z[i] : 0.0;
+    const double nx = valid ? src.nx[i] : 0.0, ny = valid ? src.ny[i] : 0.0, nz = valid ? src.nz[i] : 0.0;
+    double acc[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++) acc[q] = 0.0;
+    const int t_begin = blockIdx.y * chunk;
+    const int t_end = min(n_tgt, t_begin + chunk);
+    for (int base = t_begin; base < t_end; base += TILE) {
+        __syncthreads();
+        const int t = base + tid;
+        if (t < t_end) {
+            tile[tid] = tg[t];
+        } else {
+            TgtRec64 z{};
+            tile[tid] = z;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int s = 0; s < TILE; s++) {
+            const TgtRec64 &r = tile[s];
+            double rev = __builtin_fma(pz, r.uz, __builtin_fma(py, r.uy, px * r.ux));
+            double f = __builtin_amdgcn_fract(rev);
+            double sn, cs;
+            sincospi(2.0 * f, &sn, &cs);
+            double w = __builtin_fma(nz, r.rz, __builtin_fma(ny, r.ry, nx * r.rx));
+            cs = (w < 0.0) ? 0.0 : cs;
+            sn = (w < 0.0) ? 0.0 : sn;
+            const double wc = w * cs, ws = w * sn;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const double ar = r.a[2 * a], ai = r.a[2 * a + 1];
+                acc[2 * a] = __builtin_fma(ar, cs, acc[2 * a]);
+                acc[2 * a] = __builtin_fma(-ai, sn, acc[2 * a]);
+                acc[2 * a + 1] = __builtin_fma(ar, sn, acc[2 * a + 1]);
+                acc[2 * a + 1] = __builtin_fma(ai, cs, acc[2 * a + 1]);
+                acc[6 + 2 * a] = __builtin_fma(ar, wc, acc[6 + 2 * a]);
+                acc[6 + 2 * a] = __builtin_fma(-ai, ws, acc[6 + 2 * a]);
+                acc[7 + 2 * a] = __builtin_fma(ar, ws, acc[7 + 2 * a]);
+                acc[7 + 2 * a] = __builtin_fma(ai, wc, acc[7 + 2 * a]);
+            }
+        }
+    }
+    if (valid) {
+#pragma unroll
+        for (int q = 0; q < 12; q++) partial[((size_t)blockIdx.y * 12 + q) * n_src + i] = acc[q];
+    }
+}
+
+// Incident H at each dish point and its directional derivative along the dish
+// normal, from the feed's packed FP64 records (w*A*J, w*A*M): fieldAtPoint's
+// H integrand (Propagation.h:1084-1119) and its derivative with respect to the
+// target position moving along n.  Output rows [12][n]: J = 2 pref n x H
+// (re xyz, im xyz), then dJ = 2 pref n x dH.  FP64 throughout.
+__global__ __launch_bounds__(256) void k_adj_currents(const SrcRec64 *__restrict__ feed, int n_feed,
+                                                GridPtrs dish, int n_dish, double k, double tdir,
+                                                double pref, double *__restrict__ jd) {
+    __shared__ SrcRec64 tile[TILE];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * TILE + tid;
+    const bool valid = i < n_dish;
+    const double px = valid ? dish.x[i] : 1.0e6, py = valid ? dish.y[i] : 1.0e6,
+                 pz = valid ? dish.z[i] : 1.0e6;
+    const double n[3] = {valid ? dish.nx[i] : 0.0, valid ? dish.ny[i] : 0.0, valid ? dish.nz[i] : 1.0};
+    double hf[6] = {0, 0, 0, 0, 0, 0};   // H: re xyz, im xyz
+    double dh[6] = {0, 0, 0, 0, 0, 0};   // dH
+    const double rev_scale = tdir * k / kTwoPi;
+    for (int base = 0; base < n_feed; base += TILE) {
+        __syncthreads();
+        const int s = base + tid;
+        if (s < n_feed) {
+            tile[tid] = feed[s];
+        } else {
+            SrcRec64 z{};
+            z.x = 1.0e7; z.y = 1.0e7; z.z = 1.0e7;
+            tile[tid] = z;
+        }
+        __syncthreads();
+        const int cnt = min(TILE, n_feed - base);
+#pragma unroll 1
+        for (int s2 = 0; s2 < cnt; s2++) {
+            const SrcRec64 &r = tile[s2];
+            const double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+            const double R = sqrt(dx * dx + dy * dy + dz * dz);
+            const double Ri = 1.0 / R;
+            const double h[3] = {dx * Ri, dy * Ri, dz * Ri};
+            if (r.nx * h[0] + r.ny * h[1] + r.nz * h[2] < 0.0) continue;   // shadowed
+            const double nu = n[0] * h[0] + n[1] * h[1] + n[2] * h[2];
+            const double dh_[3] = {(n[0] - nu * h[0]) * Ri, (n[1] - nu * h[1]) * Ri, (n[2] - nu * h[2]) * Ri};
+            const double q = Ri / k, q2 = q * q, q3 = q2 * q;
+            const double dq = -k * q2 * nu;
+            // s-coefficients and their q-derivatives as (re, im)
+            const double s1r = tdir * q2, s1i = q3 - q;
+            const double s2r = -3.0 * tdir * q2, s2i = q - 3.0 * q3;
+            const double s3r = -q2, s3i = tdir * q;
+            const double d1r = 2.0 * tdir * q * dq, d1i = (3.0 * q2 - 1.0) * dq;
+            const double d2r = -6.0 * tdir * q * dq, d2i = (1.0 - 9.0 * q2) * dq;
+            const double d3r = -2.0 * q * dq, d3i = tdir * dq;
+            double sn, cs;
+            sincospi(2.0 * __builtin_amdgcn_fract(R * rev_scale), &sn, &cs);   // G = exp(j t k R)
+            // dG = j t k nu G
+            const double tkn = tdir * k * nu;
+            const double dgr = -tkn * sn, dgi = tkn * cs;
+            const double *J = r.c, *M = r.c + 6;   // (re, im) interleaved per axis
+            double mdh_r = 0, mdh_i = 0, mh_r = 0, mh_i = 0;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                mh_r += M[2 * a] * h[a]; mh_i += M[2 * a + 1] * h[a];
+                mdh_r += M[2 * a] * dh_[a]; mdh_i += M[2 * a + 1] * dh_[a];
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const int b = (a + 1) % 3, c = (a + 2) % 3;
+                const double jxh_r = J[2 * b] * h[c] - J[2 * c] * h[b];
+                const double jxh_i = J[2 * b + 1] * h[c] - J[2 * c + 1] * h[b];
+                const double jxd_r = J[2 * b] * dh_[c] - J[2 * c] * dh_[b];
+                const double jxd_i = J[2 * b + 1] * dh_[c] - J[2 * c + 1] * dh_[b];
+                // T = M s1 + (M.h) h s2 - (J x h) s3
+                const double v2r = mh_r * h[a], v2i = mh_i * h[a];
+                double Tr = (M[2 * a] * s1r - M[2 * a + 1] * s1i) + (v2r * s2r - v2i * s2i) -
+                            (jxh_r * s3r - jxh_i * s3i);
+                double Ti = (M[2 * a] * s1i + M[2 * a + 1] * s1r) + (v2r * s2i + v2i * s2r) -
+                            (jxh_r * s3i + jxh_i * s3r);
+                // dT = M ds1 + [(M.dh) h + (M.h) dh] s2 + (M.h) h ds2 - (J x dh) s3 - (J x h) ds3
+                const double w2r = mdh_r * h[a] + mh_r * dh_[a], w2i = mdh_i * h[a] + mh_i * dh_[a];
+                double dTr = (M[2 * a] * d1r - M[2 * a + 1] * d1i) + (w2r * s2r - w2i * s2i) +
+                             (v2r * d2r - v2i * d2i) - (jxd_r * s3r - jxd_i * s3i) -
+                             (jxh_r * d3r - jxh_i * d3i);
+                double dTi = (M[2 * a] * d1i + M[2 * a + 1] * d1r) + (w2r * s2i + w2i * s2r) +
+                             (v2r * d2i + v2i * d2r) - (jxd_r * s3i + jxd_i * s3r) -
+                             (jxh_r * d3i + jxh_i * d3r);
+                hf[a] += cs * Tr - sn * Ti;
+                hf[3 + a] += cs * Ti + sn * Tr;
+                // d(G T) = dG T + G dT
+                dh[a] += (dgr * Tr - dgi * Ti) + (cs * dTr - sn * dTi);
+                dh[3 + a] += (dgr * Ti + dgi * Tr) + (cs * dTi + sn * dTr);
+            }
+        }
+    }
+    if (valid) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const int b = (a + 1) % 3, c = (a + 2) % 3;
+            const double f = 2.0 * pref;
+            jd[(size_t)a * n_dish + i] = f * (n[b] * hf[c] - n[c] * hf[b]);
+            jd[(size_t)(3 + a) * n_dish + i] = f * (n[b] * hf[3 + c] - n[c] * hf[3 + b]);
+            jd[(size_t)(6 + a) * n_dish + i] = f * (n[b] * dh[c] - n[c] * dh[b]);
+            jd[(size_t)(9 + a) * n_dish + i] = f * (n[b] * dh[3 + c] - n[c] * dh[3 + b]);
+        }
+    }
+}
+
+// dL/dd_i = c_i [ Re(B_i . dJ_i) + t k Im(J_i . G_i) ]
+__global__ __launch_bounds__(256) void k_adj_combine(const double *__restrict__ partial, int splits,
+                                                     GridPtrs dish, int n, int gmode, int ncx,
+                                                     int ncy, const double *__restrict__ jd,
+                                                     double k, double tdir, double *__restrict__ grad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double S[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++) {
+        double v = 0.0;
+        for (int s = 0; s < splits; s++) v += partial[((size_t)s * 12 + q) * n + i];
+        S[q] = v;
+    }
+    double re_bdj = 0.0, im_jg = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double br = S[2 * a], bi = S[2 * a + 1], gr = S[6 + 2 * a], gi = S[7 + 2 * a];
+        const double jr = jd[(size_t)a * n + i], ji = jd[(size_t)(3 + a) * n + i];
+        const double dr = jd[(size_t)(6 + a) * n + i], di = jd[(size_t)(9 + a) * n + i];
+        re_bdj += br * dr - bi * di;
+        im_jg += jr * gi + ji * gr;
+    }
+    const double c = trapezoid_weight(i, gmode, ncx, ncy) * dish.area[i];
+    grad[i] = c * (re_bdj + tdir * k * im_jg);
+}
+
+__global__ __launch_bounds__(256) void k_displace(GridPtrs g0, const double *__restrict__ d, int n,
+                                                  GridPtrs out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double di = d[i];
+    out.x[i] = g0.x[i] + di * g0.nx[i];
+    out.y[i] = g0.y[i] + di * g0.ny[i];
+    out.z[i] = g0.z[i] + di * g0.nz[i];
+    out.nx[i] = g0.nx[i];
+    out.ny[i] = g0.ny[i];
+    out.nz[i] = g0.nz[i];
+    out.area[i] = g0.area[i];
+}
+
+}  // namespace
+
+int launch_displace(const GridPtrs &g0, const double *d, int n, const GridPtrs &out, hipStream_t st) {
+    PYPO_REQUIRE(n > 0 && d, ERR_ARG, "displace: empty grid or NULL displacement");
+    hipLaunchKernelGGL(k_displace, dim3((n + 255) / 256), dim3(256), 0, st, g0, d, n, out);
+    PYPO_CHECK_LAUNCH();
+    return 0;
+}
+
+namespace {
+
+// Target splits of the reverse sum: enough workgroups to fill the chip,
+// at least four 256-target tiles per split.
+struct AdjPlan {
+    int blocks, splits, chunk;
+    size_t tg_bytes, feed_bytes, part_bytes, jd_bytes;
+};
+
+AdjPlan adj_plan(int precision, int n_feed, int n_dish, int n_tgt) {
+    AdjPlan p;
+    p.blocks = (n_dish + TILE - 1) / TILE;
+    int splits = (2048 + p.blocks - 1) / p.blocks;
+    const int max_splits = (n_tgt + 4 * TILE - 1) / (4 * TILE);
+    splits = std::max(1, std::min(splits, max_splits));
+    p.chunk = ((n_tgt + splits - 1) / splits + TILE - 1) / TILE * TILE;
+    p.splits = (n_tgt + p.chunk - 1) / p.chunk;
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    p.tg_bytes = up((size_t)n_tgt * (precision == PYPO_PREC_FP64 ? sizeof(TgtRec64) : sizeof(TgtRec32)));
+    p.feed_bytes = up(packed_bytes(PYPO_MODE_JM, PYPO_PREC_FP64, n_feed));
+    p.part_bytes = up((size_t)p.splits * 12 * n_dish * sizeof(double));
+    p.jd_bytes = up((size_t)12 * n_dish * sizeof(double));
+    return p;
+}
+
+}  // namespace
+
+size_t adjoint_workspace_bytes(int precision, int n_feed, int n_dish, int n_tgt) {
+    AdjPlan p = adj_plan(precision, n_feed, n_dish, n_tgt);
+    return p.tg_bytes + p.feed_bytes + p.part_bytes + p.jd_bytes;
+}
+
+int launch_ff_power_grad(int precision, const GridPtrs &dish, int n_dish, int gmode, int ncx, int ncy,
+                         const GridPtrs &feed, int n_feed, int fgmode, int fncx,
+                         int fncy, const Bundle12 &feed_cur, const GridPtrs &ff, int n_tgt,
+                         const Bundle12 &E, const double *g, double k, double tdir, double *grad,
+                         void *ws, size_t ws_bytes, hipStream_t st) {
+    PYPO_REQUIRE(n_dish > 0 && n_feed > 0 && n_tgt > 0, ERR_ARG, "adjoint: empty grid");
+    PYPO_REQUIRE((long long)ncx * ncy == n_dish && (long long)fncx * fncy == n_feed, ERR_ARG,
+                 "adjoint: n_cells do not match the grid sizes");
+    PYPO_REQUIRE(ws_bytes >= adjoint_workspace_bytes(precision, n_feed, n_dish, n_tgt), ERR_ARG,
+                 "adjoint: workspace too small");
+    const PhysConsts pc = phys_consts(k, 1.0);
+    const bool fp64 = precision == PYPO_PREC_FP64;
+    const AdjPlan p = adj_plan(precision, n_feed, n_dish, n_tgt);
+    char *w = (char *)ws;
+    void *tg = w;
+    void *fpk = w + p.tg_bytes;
+    double *part = (double *)(w + p.tg_bytes + p.feed_bytes);
+    double *jd = (double *)(w + p.tg_bytes + p.feed_bytes + p.part_bytes);
+    const int blocks = p.blocks, splits = p.splits, chunk = p.chunk;
+    const double rev_scale = -tdir * k / kTwoPi;
+    const int tb = (n_tgt + 255) / 256;
+    if (fp64)
+        hipLaunchKernelGGL(k_adj_targets<TgtRec64>, dim3(tb), dim3(256), 0, st, ff.x, ff.y, E, g, n_tgt,
+                           rev_scale, pc.prefactor, (TgtRec64 *)tg);
+    else
+        hipLaunchKernelGGL(k_adj_targets<TgtRec32>, dim3(tb), dim3(256), 0, st, ff.x, ff.y, E, g, n_tgt,
+                           rev_scale, pc.prefactor, (TgtRec32 *)tg);
+    PYPO_CHECK_LAUNCH();
+    int err = launch_pack_sources(PYPO_PREC_FP64, feed, n_feed, feed_cur, fgmode, fncx, fncy, fpk, st);
+    if (err) return err;
+    hipLaunchKernelGGL(k_adj_currents, dim3(blocks), dim3(TILE), 0, st, (const SrcRec64 *)fpk, n_feed,
+                       dish, n_dish, k, tdir, pc.prefactor, jd);
+    PYPO_CHECK_LAUNCH();
+    if (fp64)
+        hipLaunchKernelGGL(k_adj_reverse64, dim3(blocks, splits), dim3(TILE), 0, st, (const TgtRec64 *)tg,
+                           n_tgt, chunk, dish, n_dish, part);
+    else
+        hipLaunchKernelGGL(k_adj_reverse32, dim3(blocks, splits), dim3(TILE), 0, st, (const TgtRec32 *)tg,
+                           n_tgt, chunk, dish, n_dish, part);
+    PYPO_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_adj_combine, dim3(blocks), dim3(256), 0, st, part, splits, dish, n_dish, gmode,
+                       ncx, ncy, jd, k, tdir, grad);
+    PYPO_CHECK_LAUNCH();
+    return 0;
+}
+
+}  // namespace pypo

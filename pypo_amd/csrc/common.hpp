@@ -165,7 +165,7 @@ constexpr double kTwoPi = 6.283185307179586476925286766559;
 void *pool_get(int slot, size_t bytes, int *err);
 enum PoolSlot {
     SLOT_SRC_GRID = 0, SLOT_TGT_GRID, SLOT_CURRENTS, SLOT_RESULT, SLOT_PACKED,
-    SLOT_PARTIAL, SLOT_ERRS_S, SLOT_ERRS_T, SLOT_RAYS, SLOT_RT_ERRS,
+    SLOT_PARTIAL, SLOT_ERRS_S, SLOT_ERRS_T, SLOT_RAYS, SLOT_RT_ERRS, SLOT_ADJ,
     SLOT_COUNT
 };
 
@@ -221,6 +221,18 @@ void sweep_destroy(Sweep *s);
 struct SweepPlanParams {
     int tpl, chunk_tiles, wgs_per_cu;
 };
+
+// Adjoint of the far-field power (adjoint.hip).  out = g0 + d n (points
+// along their normals; normals and areas copied).
+int launch_displace(const GridPtrs &g0, const double *d, int n, const GridPtrs &out, hipStream_t st);
+size_t adjoint_workspace_bytes(int precision, int n_feed, int n_dish, int n_tgt);
+// grad[i] = dL/dd_i for L with dL/d|E_t|^2 = g[t]; dish is the displaced
+// grid, E the far field of its JM currents.  All pointers device memory.
+int launch_ff_power_grad(int precision, const GridPtrs &dish, int n_dish, int gmode, int ncx, int ncy,
+                         const GridPtrs &feed, int n_feed, int fgmode, int fncx,
+                         int fncy, const Bundle12 &feed_cur, const GridPtrs &ff, int n_tgt,
+                         const Bundle12 &E, const double *g, double k, double tdir, double *grad,
+                         void *ws, size_t ws_bytes, hipStream_t st);
 
 // Kernel timing of the last timed launch (HIP events on the launch stream).
 void record_kernel_ms(double ms);
