@@ -17,7 +17,12 @@ figure, the ray tracer's rays/s on the Cassegrain configuration, and the
 CPU baseline timed on the host (reference CPU library where it was built,
 else this project's restatement) on the same subset.
 
+Secondary objects in the same line: "sweep" (the 256-instance batched sweep,
+autotuned, hipGraph replay) and "adjoint" (forward + backward of the
+far-field power gradient on the 601^2 surface-error workload).
+
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fp64] [--no-rt]
+                       [--no-sweep] [--no-adjoint]
 """
 
 import argparse
@@ -137,16 +142,110 @@ def run_rt(torch, D):
     sec = G.hyperboloid({"pmode": "focus", "focus_1": [0, 0, F_DISH], "focus_2": [0, 0, -300.0],
                          "ecc": 1.1, "gmode": "xy", "lims_x": [-150, 150], "lims_y": [-150, 150],
                          "gridsize": [3, 3]})
+    from pypo_amd import bind as B
+    lib = B.loadHIPlib()
     t_fr = D.to_device(fr)
-    D.trace([prim, sec], t_fr, tol=1e-12, t0=1000.0)
+    outs = D.trace([prim, sec], t_fr, tol=1e-12, t0=1000.0)
     torch.cuda.synchronize()
     reps = 5
     t = time.perf_counter()
     for _ in range(reps):
-        D.trace([prim, sec], t_fr, tol=1e-12, t0=1000.0)
+        outs = D.trace([prim, sec], t_fr, tol=1e-12, t0=1000.0)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t) / reps
-    return X.size / dt, dt * 1e3, X.size
+    lib.pypo_hip_set_timing(1)
+    kms = []
+    for _ in range(3):
+        D.trace([prim, sec], t_fr, tol=1e-12, t0=1000.0)
+        torch.cuda.synchronize()
+        kms.append(lib.pypo_hip_last_kernel_ms())
+    lib.pypo_hip_set_timing(0)
+    del outs
+    return X.size / dt, dt * 1e3, X.size, float(np.mean(kms))
+
+
+def run_sweep(torch, log):
+    """Batched sweep: 64 frequencies (80-120 GHz) x 4 feed offsets = 256
+    JM -> FF instances, dish grids 201^2 .. 401^2 (uneven), one 181^2 far
+    field, each stage a persistent launch, the run a replayed hipGraph; launch
+    shape from the autotuner (fresh tuning, no cache)."""
+    from pypo_amd import autotune as AT
+    from pypo_amd import sweep as SW
+    feeds, dishes, ks, curs, ff, meta = SW.frequency_offset_sweep()
+    t = time.perf_counter()
+    sw = SW.Sweep(feeds, dishes, ks, curs, ff, use_graph=False)
+    setup_s = time.perf_counter() - t
+    t = time.perf_counter()
+    best, secs, table = AT.tune(sw, use_cache=False, use_graph=True)
+    tune_s = time.perf_counter() - t
+    sw.run()
+    torch.cuda.synchronize()
+    reps = 5
+    t = time.perf_counter()
+    for _ in range(reps):
+        sw.run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    info = sw.info()
+    pairs = info["pairs_jm"] + info["pairs_ff"]
+    worst = max(table, key=lambda r: r["seconds"])
+    sw.close()
+    log(f"sweep {len(feeds)} instances: {dt * 1e3:.1f} ms/run ({pairs / dt:.3e} pairs/s), tuned {best} "
+        f"in {tune_s:.1f} s")
+    return {"instances": len(feeds), "pairs_per_run": pairs, "ms_per_run": round(dt * 1e3, 2),
+            "pairs_per_s": pairs / dt, "launch": "hipGraph replay of 2 persistent stages",
+            "tuned": best, "tuning_s": round(tune_s, 1), "setup_s": round(setup_s, 1),
+            "slowest_candidate_ms": round(worst["seconds"] * 1e3, 1),
+            "workload": "64 freqs 80-120 GHz x offsets (0,2,-2,4) mm; paraboloid D=1000 f=400 uv "
+                        "201^2..401^2; feed 25x25; AoE 181^2 +-1deg"}
+
+
+def run_adjoint(torch, log):
+    """Differentiated far field: 601^2 dish (D = 3 m, f = 1.2 m, lambda = 1
+    mm) with a 36-mode Zernike displacement of RMS 20 um, 201^2 far field over
+    +-0.2 deg, loss sum (|E|^2 - P_ideal)^2; forward (JM + FF) and backward
+    (currents recomputed + reverse far-field sum), mixed precision."""
+    from pypo_amd import adjoint as PA
+    feed, cur, dish, ff, k, radius = PA.surface_error_workload()
+    prob = PA.ShapeProblem(feed, cur, dish, ff, k, precision="mixed")
+    g0 = prob.g_dish0.cpu().numpy()
+    d0 = torch.tensor(PA.zernike_displacement(g0[0], g0[1], radius, sigma=0.02, seed=2024), device="cuda")
+    T = (prob.far_field(torch.zeros_like(d0))[0:6] ** 2).sum(0)
+
+    def fwd_bwd():
+        d = d0.clone().requires_grad_(True)
+        P = prob.power(d)
+        loss = ((P - T) ** 2).sum() / T.max() ** 2
+        loss.backward()
+        return d.grad
+
+    fwd_bwd()
+    torch.cuda.synchronize()
+    reps = 5
+    t = time.perf_counter()
+    for _ in range(reps):
+        prob.far_field(d0)
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t) / reps
+    t = time.perf_counter()
+    for _ in range(reps):
+        g = fwd_bwd()
+    torch.cuda.synchronize()
+    t_fb = (time.perf_counter() - t) / reps
+    E64 = prob.far_field(d0, precision="fp64")
+    P64 = (E64[0:6] ** 2).sum(0)
+    g64 = prob.power_grad(d0, E64, 2 * (P64 - T) / T.max() ** 2, precision="fp64")
+    rel = (torch.linalg.norm(g - g64) / torch.linalg.norm(g64)).item()
+    nf, nd, nt = prob.n_feed, prob.n_dish, prob.n_ff
+    pairs_b = nf * nd + nd * nt
+    t_b = t_fb - t_f
+    log(f"adjoint: forward {t_f * 1e3:.1f} ms, forward+backward {t_fb * 1e3:.1f} ms, "
+        f"mixed vs fp64 gradient {rel:.2e}")
+    return {"ms_forward": round(t_f * 1e3, 2), "ms_forward_backward": round(t_fb * 1e3, 2),
+            "backward_pairs": pairs_b, "backward_pairs_per_s": pairs_b / t_b,
+            "grad_mixed_vs_fp64_rel": rel,
+            "workload": "feed 33x33 -> paraboloid D=3000 f=1200 uv 601x601, 36 Zernike modes RMS 20um "
+                        "-> AoE 201x201 +-0.2deg, lambda=1mm; gradient w.r.t. 361201 normal displacements"}
 
 
 def cpu_baseline(dish, ff, cur_dish, sub_rows):
@@ -179,6 +278,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-rt", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--profile-adjoint", action="store_true",
+                    help="only the adjoint forward + backward (for rocprofv3 kernel statistics)")
+    ap.add_argument("--profile-sweep", action="store_true",
+                    help="only the batched sweep at its tuned launch shape (for rocprofv3)")
     ap.add_argument("--profile", action="store_true",
                     help="one launch of each kernel plus a 512 MiB device copy (counter calibration), "
                          "for rocprofv3 runs; prints no JSON line")
@@ -196,6 +301,20 @@ def main():
     if lib.pypo_hip_device_count() < 1 or not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+
+    if args.profile_adjoint:
+        run_adjoint(torch, log)
+        return
+    if args.profile_sweep:
+        from pypo_amd import sweep as SW
+        feeds, dishes, ks, curs, ff_s, _ = SW.frequency_offset_sweep()
+        sw = SW.Sweep(feeds, dishes, ks, curs, ff_s, tpl=2, chunk_tiles=4096, wgs_per_cu=4, use_graph=True)
+        for _ in range(3):
+            sw.run()
+        torch.cuda.synchronize()
+        sw.close()
+        log("sweep profile run done")
+        return
 
     w0 = feed_waist()
     feed, dish, ff = surfaces()
@@ -288,11 +407,17 @@ def main():
         log(f"cpu {cpu['value']:.3e} pairs/s; errors {res}")
 
     if not args.no_rt:
-        rays_s, rt_ms, nrays = run_rt(torch, D)
+        rays_s, rt_ms, nrays, rt_kms = run_rt(torch, D)
         res["rt_rays_per_s"] = rays_s
         res["rt_ms_cassegrain"] = round(rt_ms, 3)
-        res["rt_hbm_GBps"] = round(nrays * (48 + 2 * 48) / (rt_ms * 1e-3) / 1e9, 1)
-        log(f"rt {rays_s:.3e} rays/s ({rt_ms:.2f} ms for {nrays} rays x 2 surfaces)")
+        res["rt_kernel_ms"] = round(rt_kms, 3)
+        res["rt_hbm_GBps"] = round(nrays * (48 + 2 * 48) / (rt_kms * 1e-3) / 1e9, 1)
+        log(f"rt {rays_s:.3e} rays/s ({rt_ms:.2f} ms wall, kernel {rt_kms:.3f} ms, for {nrays} rays x 2 surfaces)")
+
+    if not args.no_sweep:
+        res["sweep"] = run_sweep(torch, log)
+    if not args.no_adjoint:
+        res["adjoint"] = run_adjoint(torch, log)
 
     line = {
         "metric": "PO pair-interactions/s JM->FF (mixed FP32/FP64 vs FP64) at <=1e-5 peak err; rays/s",
