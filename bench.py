@@ -147,12 +147,15 @@ def run_rt(torch, D):
     t_fr = D.to_device(fr)
     outs = D.trace([prim, sec], t_fr, tol=1e-12, t0=1000.0)
     torch.cuda.synchronize()
-    reps = 5
-    t = time.perf_counter()
-    for _ in range(reps):
+    # median of individually synchronised calls: one trace is ~0.4 ms, so a
+    # single host-side stall would otherwise dominate a short mean
+    walls = []
+    for _ in range(10):
+        t = time.perf_counter()
         outs = D.trace([prim, sec], t_fr, tol=1e-12, t0=1000.0)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t) / reps
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t)
+    dt = float(np.median(walls))
     lib.pypo_hip_set_timing(1)
     kms = []
     for _ in range(3):

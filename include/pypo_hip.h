@@ -246,16 +246,24 @@ int pypo_hip_displace_dev(const reflcontainer *base, const double *d, reflcontai
  * n_cells); ff_fields: the FF result, on the ff grid, of the JM currents
  * that feed_currents induce on the dish, both computed with the same k and
  * t_direction.  The dish currents and their derivatives are recomputed
- * here from the feed (FP64), not read.  All arrays device memory;
- * precision PYPO_PREC_MIXED or PYPO_PREC_FP64 selects the reverse
- * far-field sum's arithmetic.  Asynchronous on `stream`, ordered like
- * pypo_hip_propagate_dev. */
+ * here from the feed, not read.  All arrays device memory; precision
+ * PYPO_PREC_FP64 runs every sum in FP64, PYPO_PREC_MIXED keeps FP64 for
+ * distances and phases and runs the integrands in FP32.  Asynchronous on
+ * `stream`, ordered like pypo_hip_propagate_dev. */
 int pypo_hip_ff_power_grad_dev(const reflcontainer *dish, int gmode, int ncx, int ncy,
                                const reflcontainer *feed,
                                int feed_gmode, int feed_ncx, int feed_ncy,
                                const c2Bundle *feed_currents, const reflcontainer *ff,
                                const c2Bundle *ff_fields, const double *dl_dp, double k,
                                double t_direction, int precision, double *grad, void *stream);
+
+/* Launch shape of the single-propagation PO sums (pypo_hip_propagate_dev and
+ * the drop-in symbols): targets per lane of the far-field kernels (1 or 2)
+ * and the workgroup count the source splits aim for.  0 restores the
+ * built-in choice (far field: 2 targets per lane above 256 targets, 8192
+ * workgroups; other modes: 1 target per lane, 2048 workgroups).  A tuning hook; results do not depend on it beyond
+ * rounding.  Returns 0, or ERR_ARG for values out of range. */
+int pypo_hip_set_sum_shape(int tpl, int want_wgs);
 
 /* Mean kernel time (ms) of the last pypo_hip_propagate_dev / trace call's
  * dominant kernel, measured with HIP events on the launch stream, when

@@ -94,6 +94,8 @@ def loadHIPlib():
         lib.pypo_hip_generate_grid_dev.restype = i
         lib.pypo_hip_trace_dev.argtypes = [P(PS.reflparams), i, P(PS.cframe), d, d, ctypes.c_void_p]
         lib.pypo_hip_trace_dev.restype = i
+        lib.pypo_hip_set_sum_shape.argtypes = [i, i]
+        lib.pypo_hip_set_sum_shape.restype = i
         RC, CB = P(PS.reflcontainer), P(PS.c2Bundle)
         lib.pypo_hip_displace_dev.argtypes = [RC, P(d), RC, ctypes.c_void_p]
         lib.pypo_hip_displace_dev.restype = i

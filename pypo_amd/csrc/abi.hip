@@ -574,6 +574,11 @@ int pypo_hip_sweep_destroy(void *handle) {
     return 0;
 }
 
+int pypo_hip_set_sum_shape(int tpl, int want_wgs) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    return set_sum_shape(tpl, want_wgs);
+}
+
 // ---- adjoint of the far-field power -------------------------------------------------
 
 int pypo_hip_displace_dev(const reflcontainer *base, const double *d, reflcontainer *out,

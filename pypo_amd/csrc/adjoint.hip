@@ -66,23 +66,34 @@ __global__ __launch_bounds__(256) void k_adj_targets(const double *__restrict__ 
     out[t] = o;
 }
 
-// Reverse far-field sum, mixed precision: lanes own sources (dish points),
-// targets stream through LDS.  Partial layout [split][12][n_src]:
-// q 0..5 = B (re/im xyz interleaved), 6..11 = G.
+// Reverse far-field sum, mixed precision: lanes own SPL sources (dish
+// points) each, targets stream through LDS; every 64-byte target record read
+// from LDS serves SPL pairs.  Phase in FP64 revolutions reduced with
+// v_fract_f64, FP32 sin/cos and FMAs, per-tile FP32 sums folded into FP64.
+// Partial layout [split][12][n_src]: q 0..5 = B (re/im xyz interleaved),
+// 6..11 = G.
+template <int SPL>
 __global__ __launch_bounds__(256) void k_adj_reverse32(const TgtRec32 *__restrict__ tg, int n_tgt,
                                                        int chunk, GridPtrs src, int n_src,
                                                        double *__restrict__ partial) {
     __shared__ TgtRec32 tile[TILE];
     const int tid = threadIdx.x;
-    const int i = blockIdx.x * TILE + tid;
-    const bool valid = i < n_src;
-    const double px = valid ? src.x[i] : 0.0, py = valid ? src.y[i] : 0.0, pz = valid ? src.z[i] : 0.0;
-    const float nx = valid ? (float)src.nx[i] : 0.f, ny = valid ? (float)src.ny[i] : 0.f,
-                nz = valid ? (float)src.nz[i] : 0.f;
-    float acc[12];
-    double tot[12];
+    double px[SPL], py[SPL], pz[SPL];
+    float nx[SPL], ny[SPL], nz[SPL];
 #pragma unroll
-    for (int q = 0; q < 12; q++) { acc[q] = 0.f; tot[q] = 0.0; }
+    for (int j = 0; j < SPL; j++) {
+        const int i = blockIdx.x * (TILE * SPL) + j * TILE + tid;
+        const bool valid = i < n_src;
+        px[j] = valid ? src.x[i] : 0.0; py[j] = valid ? src.y[i] : 0.0; pz[j] = valid ? src.z[i] : 0.0;
+        nx[j] = valid ? (float)src.nx[i] : 0.f; ny[j] = valid ? (float)src.ny[i] : 0.f;
+        nz[j] = valid ? (float)src.nz[i] : 0.f;
+    }
+    float acc[SPL][12];
+    double tot[SPL][12];
+#pragma unroll
+    for (int j = 0; j < SPL; j++)
+#pragma unroll
+        for (int q = 0; q < 12; q++) { acc[j][q] = 0.f; tot[j][q] = 0.0; }
     const int t_begin = blockIdx.y * chunk;
     const int t_end = min(n_tgt, t_begin + chunk);
     for (int base = t_begin; base < t_end; base += TILE) {
@@ -98,33 +109,44 @@ __global__ __launch_bounds__(256) void k_adj_reverse32(const TgtRec32 *__restric
 #pragma unroll 2
         for (int s = 0; s < TILE; s++) {
             const TgtRec32 &r = tile[s];
-            double rev = __builtin_fma(pz, r.uz, __builtin_fma(py, r.uy, px * r.ux));
-            float f = (float)__builtin_amdgcn_fract(rev);
-            float sn, cs;
-            sincos_rev_f32(f, sn, cs);
-            float w = __builtin_fmaf(nz, r.rz, __builtin_fmaf(ny, r.ry, nx * r.rx));
-            cs = (w < 0.f) ? 0.f : cs;
-            sn = (w < 0.f) ? 0.f : sn;
-            const float wc = w * cs, ws = w * sn;
+            const float a[6] = {r.a[0], r.a[1], r.a[2], r.a[3], r.a[4], r.a[5]};
 #pragma unroll
-            for (int a = 0; a < 3; a++) {
-                const float ar = r.a[2 * a], ai = r.a[2 * a + 1];
-                acc[2 * a] = __builtin_fmaf(ar, cs, acc[2 * a]);
-                acc[2 * a] = __builtin_fmaf(-ai, sn, acc[2 * a]);
-                acc[2 * a + 1] = __builtin_fmaf(ar, sn, acc[2 * a + 1]);
-                acc[2 * a + 1] = __builtin_fmaf(ai, cs, acc[2 * a + 1]);
-                acc[6 + 2 * a] = __builtin_fmaf(ar, wc, acc[6 + 2 * a]);
-                acc[6 + 2 * a] = __builtin_fmaf(-ai, ws, acc[6 + 2 * a]);
-                acc[7 + 2 * a] = __builtin_fmaf(ar, ws, acc[7 + 2 * a]);
-                acc[7 + 2 * a] = __builtin_fmaf(ai, wc, acc[7 + 2 * a]);
+            for (int j = 0; j < SPL; j++) {
+                double rev = __builtin_fma(pz[j], r.uz, __builtin_fma(py[j], r.uy, px[j] * r.ux));
+                float f = (float)__builtin_amdgcn_fract(rev);
+                float sn, cs;
+                sincos_rev_f32(f, sn, cs);
+                float w = __builtin_fmaf(nz[j], r.rz, __builtin_fmaf(ny[j], r.ry, nx[j] * r.rx));
+                cs = (w < 0.f) ? 0.f : cs;
+                sn = (w < 0.f) ? 0.f : sn;
+                const float wc = w * cs, ws = w * sn;
+                float *c = acc[j];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const float ar = a[2 * q], ai = a[2 * q + 1];
+                    c[2 * q] = __builtin_fmaf(ar, cs, c[2 * q]);
+                    c[2 * q] = __builtin_fmaf(-ai, sn, c[2 * q]);
+                    c[2 * q + 1] = __builtin_fmaf(ar, sn, c[2 * q + 1]);
+                    c[2 * q + 1] = __builtin_fmaf(ai, cs, c[2 * q + 1]);
+                    c[6 + 2 * q] = __builtin_fmaf(ar, wc, c[6 + 2 * q]);
+                    c[6 + 2 * q] = __builtin_fmaf(-ai, ws, c[6 + 2 * q]);
+                    c[7 + 2 * q] = __builtin_fmaf(ar, ws, c[7 + 2 * q]);
+                    c[7 + 2 * q] = __builtin_fmaf(ai, wc, c[7 + 2 * q]);
+                }
             }
         }
 #pragma unroll
-        for (int q = 0; q < 12; q++) { tot[q] += (double)acc[q]; acc[q] = 0.f; }
-    }
-    if (valid) {
+        for (int j = 0; j < SPL; j++)
 #pragma unroll
-        for (int q = 0; q < 12; q++) partial[((size_t)blockIdx.y * 12 + q) * n_src + i] = tot[q];
+            for (int q = 0; q < 12; q++) { tot[j][q] += (double)acc[j][q]; acc[j][q] = 0.f; }
+    }
+#pragma unroll
+    for (int j = 0; j < SPL; j++) {
+        const int i = blockIdx.x * (TILE * SPL) + j * TILE + tid;
+        if (i < n_src) {
+#pragma unroll
+            for (int q = 0; q < 12; q++) partial[((size_t)blockIdx.y * 12 + q) * n_src + i] = tot[j][q];
+        }
     }
 }
 
@@ -286,6 +308,116 @@ __global__ __launch_bounds__(256) void k_adj_currents(const SrcRec64 *__restrict
     }
 }
 
+// Mixed-precision twin of k_adj_currents for the PYPO_PREC_MIXED path: FP64
+// difference vector, distance (v_rsq_f32 + one FP64 Newton step) and phase;
+// FP32 unit vectors, s-coefficients, their derivatives and the per-tile
+// sums (folded into FP64).  Feed records are the mixed-precision SrcRec32.
+__global__ __launch_bounds__(256) void k_adj_currents32(const SrcRec32 *__restrict__ feed, int n_feed,
+                                                        GridPtrs dish, int n_dish, double k, double tdir,
+                                                        double pref, double *__restrict__ jd) {
+    __shared__ SrcRec32 tile[TILE];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * TILE + tid;
+    const bool valid = i < n_dish;
+    const double px = valid ? dish.x[i] : 1.0e6, py = valid ? dish.y[i] : 1.0e6,
+                 pz = valid ? dish.z[i] : 1.0e6;
+    const float n[3] = {valid ? (float)dish.nx[i] : 0.f, valid ? (float)dish.ny[i] : 0.f,
+                        valid ? (float)dish.nz[i] : 1.f};
+    const double rev_scale = tdir * k / kTwoPi;   // G = exp(j t k R)
+    const float kf = (float)k, kinv = (float)(1.0 / k), tf = (float)tdir;
+    float hf[6], dh[6];      // per-tile sums: H and dH (re xyz, im xyz)
+    double HF[6], DH[6];     // totals
+#pragma unroll
+    for (int q = 0; q < 6; q++) { hf[q] = 0.f; dh[q] = 0.f; HF[q] = 0.0; DH[q] = 0.0; }
+    for (int base = 0; base < n_feed; base += TILE) {
+        __syncthreads();
+        const int s = base + tid;
+        if (s < n_feed) {
+            tile[tid] = feed[s];
+        } else {
+            SrcRec32 z{};
+            z.x = 1.0e7; z.y = 1.0e7; z.z = 1.0e7;
+            tile[tid] = z;
+        }
+        __syncthreads();
+        const int cnt = min(TILE, n_feed - base);
+#pragma unroll 1
+        for (int s2 = 0; s2 < cnt; s2++) {
+            const SrcRec32 &r = tile[s2];
+            const double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+            const double R2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+            const float y0 = __builtin_amdgcn_rsqf((float)R2);
+            const double yd = (double)y0;
+            const double tR = R2 * yd;
+            const double e = __builtin_fma(-tR, yd, 1.0);
+            const double R = __builtin_fma(0.5 * tR, e, tR);
+            float sn, cs;
+            sincos_rev_f32((float)__builtin_amdgcn_fract(R * rev_scale), sn, cs);
+            const float Ri = __builtin_fmaf(0.5f * y0, (float)e, y0);
+            const float h[3] = {(float)dx * Ri, (float)dy * Ri, (float)dz * Ri};
+            const float m = __builtin_fmaf(r.nz, h[2], __builtin_fmaf(r.ny, h[1], r.nx * h[0]));
+            cs = (m < 0.f) ? 0.f : cs;   // shadowed: G = dG = 0
+            sn = (m < 0.f) ? 0.f : sn;
+            const float nu = __builtin_fmaf(n[2], h[2], __builtin_fmaf(n[1], h[1], n[0] * h[0]));
+            const float dv[3] = {(n[0] - nu * h[0]) * Ri, (n[1] - nu * h[1]) * Ri, (n[2] - nu * h[2]) * Ri};
+            const float q = Ri * kinv, q2 = q * q, q3 = q2 * q;
+            const float dq = -kf * q2 * nu;
+            const float s1r = tf * q2, s1i = q3 - q;
+            const float s2r = -3.f * tf * q2, s2i = __builtin_fmaf(-3.f, q3, q);
+            const float s3r = -q2, s3i = tf * q;
+            const float d1r = 2.f * tf * q * dq, d1i = __builtin_fmaf(3.f, q2, -1.f) * dq;
+            const float d2r = -6.f * tf * q * dq, d2i = __builtin_fmaf(-9.f, q2, 1.f) * dq;
+            const float d3r = -2.f * q * dq, d3i = tf * dq;
+            const float tkn = tf * kf * nu;
+            const float dgr = -tkn * sn, dgi = tkn * cs;
+            const float *J = r.cj, *M = r.cm;
+            float mh_r = 0.f, mh_i = 0.f, md_r = 0.f, md_i = 0.f;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                mh_r = __builtin_fmaf(M[2 * a], h[a], mh_r); mh_i = __builtin_fmaf(M[2 * a + 1], h[a], mh_i);
+                md_r = __builtin_fmaf(M[2 * a], dv[a], md_r); md_i = __builtin_fmaf(M[2 * a + 1], dv[a], md_i);
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const int b = (a + 1) % 3, c = (a + 2) % 3;
+                const float xr = J[2 * b] * h[c] - J[2 * c] * h[b];
+                const float xi = J[2 * b + 1] * h[c] - J[2 * c + 1] * h[b];
+                const float yr = J[2 * b] * dv[c] - J[2 * c] * dv[b];
+                const float yi = J[2 * b + 1] * dv[c] - J[2 * c + 1] * dv[b];
+                const float vr = mh_r * h[a], vi = mh_i * h[a];
+                const float wr = md_r * h[a] + mh_r * dv[a], wi = md_i * h[a] + mh_i * dv[a];
+                const float Mr = M[2 * a], Mi = M[2 * a + 1];
+                // T = M s1 + (M.h) h s2 - (J x h) s3
+                const float Tr = (Mr * s1r - Mi * s1i) + (vr * s2r - vi * s2i) - (xr * s3r - xi * s3i);
+                const float Ti = (Mr * s1i + Mi * s1r) + (vr * s2i + vi * s2r) - (xr * s3i + xi * s3r);
+                // dT = M ds1 + [(M.dh) h + (M.h) dh] s2 + (M.h) h ds2 - (J x dh) s3 - (J x h) ds3
+                const float dTr = (Mr * d1r - Mi * d1i) + (wr * s2r - wi * s2i) + (vr * d2r - vi * d2i) -
+                                  (yr * s3r - yi * s3i) - (xr * d3r - xi * d3i);
+                const float dTi = (Mr * d1i + Mi * d1r) + (wr * s2i + wi * s2r) + (vr * d2i + vi * d2r) -
+                                  (yr * s3i + yi * s3r) - (xr * d3i + xi * d3r);
+                hf[a] += cs * Tr - sn * Ti;
+                hf[3 + a] += cs * Ti + sn * Tr;
+                dh[a] += (dgr * Tr - dgi * Ti) + (cs * dTr - sn * dTi);
+                dh[3 + a] += (dgr * Ti + dgi * Tr) + (cs * dTi + sn * dTr);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 6; q++) { HF[q] += (double)hf[q]; DH[q] += (double)dh[q]; hf[q] = 0.f; dh[q] = 0.f; }
+    }
+    if (valid) {
+        const double nd[3] = {dish.nx[i], dish.ny[i], dish.nz[i]};
+        const double f = 2.0 * pref;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const int b = (a + 1) % 3, c = (a + 2) % 3;
+            jd[(size_t)a * n_dish + i] = f * (nd[b] * HF[c] - nd[c] * HF[b]);
+            jd[(size_t)(3 + a) * n_dish + i] = f * (nd[b] * HF[3 + c] - nd[c] * HF[3 + b]);
+            jd[(size_t)(6 + a) * n_dish + i] = f * (nd[b] * DH[c] - nd[c] * DH[b]);
+            jd[(size_t)(9 + a) * n_dish + i] = f * (nd[b] * DH[3 + c] - nd[c] * DH[3 + b]);
+        }
+    }
+}
+
 // dL/dd_i = c_i [ Re(B_i . dJ_i) + t k Im(J_i . G_i) ]
 __global__ __launch_bounds__(256) void k_adj_combine(const double *__restrict__ partial, int splits,
                                                      GridPtrs dish, int n, int gmode, int ncx,
@@ -341,21 +473,26 @@ namespace {
 // Target splits of the reverse sum: enough workgroups to fill the chip,
 // at least four 256-target tiles per split.
 struct AdjPlan {
-    int blocks, splits, chunk;
+    int spl;       // dish points per lane in the reverse sum
+    int blocks;    // workgroups of one point per lane
+    int rblocks;   // reverse-sum workgroups along the dish points
+    int splits, chunk;
     size_t tg_bytes, feed_bytes, part_bytes, jd_bytes;
 };
 
 AdjPlan adj_plan(int precision, int n_feed, int n_dish, int n_tgt) {
     AdjPlan p;
+    p.spl = precision == PYPO_PREC_FP64 ? 1 : 2;
     p.blocks = (n_dish + TILE - 1) / TILE;
-    int splits = (2048 + p.blocks - 1) / p.blocks;
+    p.rblocks = (n_dish + TILE * p.spl - 1) / (TILE * p.spl);
+    int splits = (2048 + p.rblocks - 1) / p.rblocks;
     const int max_splits = (n_tgt + 4 * TILE - 1) / (4 * TILE);
     splits = std::max(1, std::min(splits, max_splits));
     p.chunk = ((n_tgt + splits - 1) / splits + TILE - 1) / TILE * TILE;
     p.splits = (n_tgt + p.chunk - 1) / p.chunk;
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     p.tg_bytes = up((size_t)n_tgt * (precision == PYPO_PREC_FP64 ? sizeof(TgtRec64) : sizeof(TgtRec32)));
-    p.feed_bytes = up(packed_bytes(PYPO_MODE_JM, PYPO_PREC_FP64, n_feed));
+    p.feed_bytes = up(packed_bytes(PYPO_MODE_JM, precision, n_feed));
     p.part_bytes = up((size_t)p.splits * 12 * n_dish * sizeof(double));
     p.jd_bytes = up((size_t)12 * n_dish * sizeof(double));
     return p;
@@ -396,17 +533,21 @@ int launch_ff_power_grad(int precision, const GridPtrs &dish, int n_dish, int gm
         hipLaunchKernelGGL(k_adj_targets<TgtRec32>, dim3(tb), dim3(256), 0, st, ff.x, ff.y, E, g, n_tgt,
                            rev_scale, pc.prefactor, (TgtRec32 *)tg);
     PYPO_CHECK_LAUNCH();
-    int err = launch_pack_sources(PYPO_PREC_FP64, feed, n_feed, feed_cur, fgmode, fncx, fncy, fpk, st);
+    int err = launch_pack_sources(precision, feed, n_feed, feed_cur, fgmode, fncx, fncy, fpk, st);
     if (err) return err;
-    hipLaunchKernelGGL(k_adj_currents, dim3(blocks), dim3(TILE), 0, st, (const SrcRec64 *)fpk, n_feed,
-                       dish, n_dish, k, tdir, pc.prefactor, jd);
+    if (fp64)
+        hipLaunchKernelGGL(k_adj_currents, dim3(blocks), dim3(TILE), 0, st, (const SrcRec64 *)fpk, n_feed,
+                           dish, n_dish, k, tdir, pc.prefactor, jd);
+    else
+        hipLaunchKernelGGL(k_adj_currents32, dim3(blocks), dim3(TILE), 0, st, (const SrcRec32 *)fpk,
+                           n_feed, dish, n_dish, k, tdir, pc.prefactor, jd);
     PYPO_CHECK_LAUNCH();
     if (fp64)
-        hipLaunchKernelGGL(k_adj_reverse64, dim3(blocks, splits), dim3(TILE), 0, st, (const TgtRec64 *)tg,
-                           n_tgt, chunk, dish, n_dish, part);
+        hipLaunchKernelGGL(k_adj_reverse64, dim3(p.rblocks, splits), dim3(TILE), 0, st,
+                           (const TgtRec64 *)tg, n_tgt, chunk, dish, n_dish, part);
     else
-        hipLaunchKernelGGL(k_adj_reverse32, dim3(blocks, splits), dim3(TILE), 0, st, (const TgtRec32 *)tg,
-                           n_tgt, chunk, dish, n_dish, part);
+        hipLaunchKernelGGL(k_adj_reverse32<2>, dim3(p.rblocks, splits), dim3(TILE), 0, st,
+                           (const TgtRec32 *)tg, n_tgt, chunk, dish, n_dish, part);
     PYPO_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_adj_combine, dim3(blocks), dim3(256), 0, st, part, splits, dish, n_dish, gmode,
                        ncx, ncy, jd, k, tdir, grad);

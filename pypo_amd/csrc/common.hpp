@@ -192,6 +192,7 @@ struct SumPlan {
     int chunk;        // sources per split
 };
 SumPlan plan_sum(int mode, int precision, int n_src, int n_tgt);
+int set_sum_shape(int tpl, int want_wgs);   // 0, 0: built-in choice
 size_t partial_bytes(int mode, const SumPlan &p, int n_tgt);
 
 // One PO propagation from packed sources to the target grid ct.  `res`

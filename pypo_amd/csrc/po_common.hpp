@@ -148,7 +148,7 @@ int launch_ff_sum(int precision, const SumPlan &p, const void *packed, int n_src
                   int n_tgt, double rev_scale, double *partial, hipStream_t st);
 int launch_ff_finish(const SumPlan &p, const GridPtrs &ct, int n_tgt, double tdir, double pref,
                      const c2Bundle &out, const double *partial, hipStream_t st);
-int launch_near_sum(int precision, const SumPlan &p, const void *packed, int n_src,
+int launch_near_sum(int mode, int precision, const SumPlan &p, const void *packed, int n_src,
                     const GridPtrs &ct, int n_tgt, double k, double tdir, double *partial,
                     hipStream_t st);
 int launch_near_finish(int mode, const SumPlan &p, const GridPtrs &ct, int n_tgt, double pref,

@@ -18,7 +18,8 @@
 namespace pypo {
 namespace {
 
-template <int TPL>
+// WANT_E = false (JM): only the H rows 6..11 of the partials are written.
+template <int TPL, bool WANT_E>
 __global__ __launch_bounds__(256) void k_near_mixed(const SrcRec32 *__restrict__ src,
                                                     const TileMeta *__restrict__ meta, int n_src,
                                                     int chunk, const double *__restrict__ tx,
@@ -58,20 +59,20 @@ __global__ __launch_bounds__(256) void k_near_mixed(const SrcRec32 *__restrict__
         __syncthreads();
         if (base + TILE < s_end) ld.load(g, base + TILE + tid, s_end);
         if (meta[base / TILE].has_m)
-            near_tile_mixed<TPL, true>(tile, px, py, pz, rev_scale, kinv, tf, acc);
+            near_tile_mixed<TPL, true, WANT_E>(tile, px, py, pz, rev_scale, kinv, tf, acc);
         else
-            near_tile_mixed<TPL, false>(tile, px, py, pz, rev_scale, kinv, tf, acc);
+            near_tile_mixed<TPL, false, WANT_E>(tile, px, py, pz, rev_scale, kinv, tf, acc);
 #pragma unroll
         for (int j = 0; j < TPL; j++)
 #pragma unroll
-            for (int q = 0; q < 12; q++) { tot[j][q] += (double)acc[j][q]; acc[j][q] = 0.f; }
+            for (int q = WANT_E ? 0 : 6; q < 12; q++) { tot[j][q] += (double)acc[j][q]; acc[j][q] = 0.f; }
     }
 #pragma unroll
     for (int j = 0; j < TPL; j++) {
         int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
         if (t < n_tgt) {
 #pragma unroll
-            for (int q = 0; q < 12; q++)
+            for (int q = WANT_E ? 0 : 6; q < 12; q++)
                 partial[((size_t)blockIdx.y * 12 + q) * n_tgt + t] = tot[j][q];
         }
     }
@@ -224,10 +225,13 @@ __global__ __launch_bounds__(256) void k_near_finish(const double *__restrict__ 
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_tgt) return;
     double S[12];
+    // the JM sums write the H rows only
+    constexpr int Q0 = (MODE == PYPO_MODE_JM) ? 6 : 0;
 #pragma unroll
     for (int q = 0; q < 12; q++) {
         double v = 0.0;
-        for (int s = 0; s < splits; s++) v += partial[((size_t)s * 12 + q) * n_tgt + t];
+        if (q >= Q0)
+            for (int s = 0; s < splits; s++) v += partial[((size_t)s * 12 + q) * n_tgt + t];
         S[q] = pref * v;
     }
     double Er[3], Ei[3], Hr[3], Hi[3];
@@ -305,7 +309,7 @@ __global__ __launch_bounds__(256) void k_near_finish(const double *__restrict__ 
 
 }  // namespace
 
-int launch_near_sum(int precision, const SumPlan &p, const void *packed, int n_src,
+int launch_near_sum(int mode, int precision, const SumPlan &p, const void *packed, int n_src,
                     const GridPtrs &ct, int n_tgt, double k, double tdir, double *partial,
                     hipStream_t st) {
     dim3 grid(p.tblocks, p.splits), block(TILE);
@@ -314,9 +318,12 @@ int launch_near_sum(int precision, const SumPlan &p, const void *packed, int n_s
     if (precision == PYPO_PREC_FP64)
         hipLaunchKernelGGL(k_near_fp64<1>, grid, block, 0, st, (const SrcRec64 *)packed, meta, n_src,
                            p.chunk, ct.x, ct.y, ct.z, n_tgt, k, tdir, partial);
+    else if (mode == PYPO_MODE_JM)
+        hipLaunchKernelGGL((k_near_mixed<1, false>), grid, block, 0, st, (const SrcRec32 *)packed, meta,
+                           n_src, p.chunk, ct.x, ct.y, ct.z, n_tgt, k, tdir, partial);
     else
-        hipLaunchKernelGGL(k_near_mixed<1>, grid, block, 0, st, (const SrcRec32 *)packed, meta, n_src,
-                           p.chunk, ct.x, ct.y, ct.z, n_tgt, k, tdir, partial);
+        hipLaunchKernelGGL((k_near_mixed<1, true>), grid, block, 0, st, (const SrcRec32 *)packed, meta,
+                           n_src, p.chunk, ct.x, ct.y, ct.z, n_tgt, k, tdir, partial);
     PYPO_CHECK_LAUNCH();
     return 0;
 }

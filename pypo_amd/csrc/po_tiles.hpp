@@ -61,7 +61,10 @@ __device__ __forceinline__ void ff_tile_mixed(const SrcRec32 *tile, const double
 }
 
 // Near field (fieldAtPoint, Propagation.h:977-1144): see po_near.hip.
-template <int TPL, bool HAS_M>
+// WANT_E = false accumulates H only (the JM mode needs nothing else); the
+// E terms and, for tiles without magnetic current, the s1/s2 factors are
+// then dead code.
+template <int TPL, bool HAS_M, bool WANT_E>
 __device__ __forceinline__ void near_tile_mixed(const SrcRec32 *tile, const double (&px)[TPL],
                                                 const double (&py)[TPL], const double (&pz)[TPL],
                                                 double rev_scale, float kinv, float tf,
@@ -121,12 +124,14 @@ __device__ __forceinline__ void near_tile_mixed(const SrcRec32 *tile, const doub
                 float jxr_i = __builtin_fmaf(cj[2 * b + 1], h[d], -cj[2 * d + 1] * h[b]);
                 float &er = acc[j][2 * a], &ei = acc[j][2 * a + 1];
                 float &hr = acc[j][6 + 2 * a], &hi = acc[j][7 + 2 * a];
-                er = __builtin_fmaf(cj[2 * a], g1r, er);
-                er = __builtin_fmaf(-cj[2 * a + 1], g1i, er);
-                er = __builtin_fmaf(h[a], Ar, er);
-                ei = __builtin_fmaf(cj[2 * a], g1i, ei);
-                ei = __builtin_fmaf(cj[2 * a + 1], g1r, ei);
-                ei = __builtin_fmaf(h[a], Ai, ei);
+                if (WANT_E) {
+                    er = __builtin_fmaf(cj[2 * a], g1r, er);
+                    er = __builtin_fmaf(-cj[2 * a + 1], g1i, er);
+                    er = __builtin_fmaf(h[a], Ar, er);
+                    ei = __builtin_fmaf(cj[2 * a], g1i, ei);
+                    ei = __builtin_fmaf(cj[2 * a + 1], g1r, ei);
+                    ei = __builtin_fmaf(h[a], Ai, ei);
+                }
                 hr = __builtin_fmaf(-jxr_r, g3r, hr);
                 hr = __builtin_fmaf(jxr_i, g3i, hr);
                 hi = __builtin_fmaf(-jxr_r, g3i, hi);
@@ -134,10 +139,12 @@ __device__ __forceinline__ void near_tile_mixed(const SrcRec32 *tile, const doub
                 if (HAS_M) {
                     float mxr_r = __builtin_fmaf(cm[2 * b], h[d], -cm[2 * d] * h[b]);
                     float mxr_i = __builtin_fmaf(cm[2 * b + 1], h[d], -cm[2 * d + 1] * h[b]);
-                    er = __builtin_fmaf(mxr_r, g3r, er);
-                    er = __builtin_fmaf(-mxr_i, g3i, er);
-                    ei = __builtin_fmaf(mxr_r, g3i, ei);
-                    ei = __builtin_fmaf(mxr_i, g3r, ei);
+                    if (WANT_E) {
+                        er = __builtin_fmaf(mxr_r, g3r, er);
+                        er = __builtin_fmaf(-mxr_i, g3i, er);
+                        ei = __builtin_fmaf(mxr_r, g3i, ei);
+                        ei = __builtin_fmaf(mxr_i, g3r, ei);
+                    }
                     hr = __builtin_fmaf(cm[2 * a], g1r, hr);
                     hr = __builtin_fmaf(-cm[2 * a + 1], g1i, hr);
                     hr = __builtin_fmaf(h[a], Br, hr);

@@ -162,21 +162,22 @@ __global__ __launch_bounds__(256) void k_sweep_near(const StageInst *__restrict_
             ld.store(reinterpret_cast<uint4 *>(tile), tid);
             __syncthreads();
             if (base + TILE < item.s1) ld.load(g, base + TILE + tid, item.s1);
+            // JM needs H only: rows 6..11 of the partials
             if (in.meta[base / TILE].has_m)
-                near_tile_mixed<TPL, true>(tile, px, py, pz, in.rev_scale, kinv, tf, acc);
+                near_tile_mixed<TPL, true, false>(tile, px, py, pz, in.rev_scale, kinv, tf, acc);
             else
-                near_tile_mixed<TPL, false>(tile, px, py, pz, in.rev_scale, kinv, tf, acc);
+                near_tile_mixed<TPL, false, false>(tile, px, py, pz, in.rev_scale, kinv, tf, acc);
 #pragma unroll
             for (int j = 0; j < TPL; j++)
 #pragma unroll
-                for (int q = 0; q < 12; q++) { tot[j][q] += (double)acc[j][q]; acc[j][q] = 0.f; }
+                for (int q = 6; q < 12; q++) { tot[j][q] += (double)acc[j][q]; acc[j][q] = 0.f; }
         }
 #pragma unroll
         for (int j = 0; j < TPL; j++) {
             int t = item.tblock * (TILE * TPL) + j * TILE + tid;
             if (t < n_tgt) {
 #pragma unroll
-                for (int q = 0; q < 12; q++)
+                for (int q = 6; q < 12; q++)
                     in.partial[((size_t)item.chunk * 12 + q) * n_tgt + t] = tot[j][q];
             }
         }
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(256) void k_sweep_jm_finish(const StageInst *__rest
     const int n = in.n_tgt;
     double S[12];
 #pragma unroll
-    for (int q = 0; q < 12; q++) {
+    for (int q = 6; q < 12; q++) {   // the JM sums write the H rows only
         double v = 0.0;
         for (int c = 0; c < in.n_chunks; c++) v += in.partial[((size_t)c * 12 + q) * n + t];
         S[q] = in.pref * v;
