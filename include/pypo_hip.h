@@ -138,6 +138,40 @@ void propagateRays(reflparams ctp, cframe *fr_in, cframe *fr_out,
  * with transform = spheric = true. */
 void generateGrid(reflparams refl, reflcontainer *container, bool transform, bool spheric);
 
+/* ---- beam initialisation: drop-in for the reference's beam library ------
+ * (src/common/InterfaceBeam.h:21-34, loaded as libpypobeam.so by
+ * src/PyPO/BindBeam.py:18-50).  Parameter structs: Structs.h:164-213;
+ * angles in radians (BindUtils.allfill_RTDict / allfill_GRTDict convert).
+ * Grids of `plane` are generated on the device with transform = spheric =
+ * true, as BeamInit.h does; results are written into the caller's host
+ * arrays; synchronous. */
+typedef struct RTDict { int nRays; int nRing; double angx0, angy0, x0, y0; } RTDict;
+typedef struct GRTDict { int nRays; double angx0, angy0, x0, y0; int seed; } GRTDict;
+typedef struct GPODict { double lam, w0x, w0y, n, E0, dxyz; double *pol; } GPODict;
+typedef struct vecGPODict { double lam, w0, z, n, power; int mode; } vecGPODict;
+typedef struct ScalarGPODict { double lam, w0x, w0y, n, E0, dxyz; } ScalarGPODict;
+
+/* Tubular ray frame, 1 + 4 nRays nRing rays (BeamInit.h:159-214); sets fr->size. */
+void makeRTframe(RTDict rdict, cframe *fr);
+/* Gaussian ray frame by rejection sampling (BeamInit.h:216-286).  Rays are
+ * drawn per ray from a counter-based generator keyed by `seed` (seed = -1:
+ * a random key).  The reference ignores the seed (its seeded generator goes
+ * out of scope, BeamInit.h:224) and draws from std::random_device, so only
+ * the distribution is comparable.  Fails when a ray finds no accepted
+ * candidate within 2^20 trials. */
+void makeGRTframe(GRTDict grdict, cframe *fr);
+/* Astigmatic Gaussian E field on `plane`, H = 0, J = 0, M = -2 n x E
+ * (BeamInit.h:288-405). */
+void makeGauss(GPODict gdict, reflparams plane, c2Bundle *res_field, c2Bundle *res_current);
+/* Vector Gaussian by the complex source point: E, H and J, M by mode
+ * 0 full / 1 PMC / 2 PEC (BeamInit.h:407-596). */
+void makeGaussBeam(vecGPODict gdict, reflparams plane, c2Bundle *res_field, c2Bundle *res_current);
+/* Scalar Gaussian (BeamInit.h:598-658). */
+void makeScalarGauss(ScalarGPODict sgdict, reflparams plane, arrC1 *res_field);
+/* Currents from fields on `rdict`'s grid, mode 0 full / 1 PMC / 2 PEC /
+ * 3 source (BeamInit.h:660-844). */
+void calcCurrents(c2Bundle *res_field, c2Bundle *res_current, reflparams rdict, int mode);
+
 /* ---- 2. extensions --------------------------------------------------- */
 
 enum pypo_hip_mode {
@@ -256,6 +290,22 @@ int pypo_hip_ff_power_grad_dev(const reflcontainer *dish, int gmode, int ncx, in
                                const c2Bundle *feed_currents, const reflcontainer *ff,
                                const c2Bundle *ff_fields, const double *dl_dp, double k,
                                double t_direction, int precision, double *grad, void *stream);
+
+/* Device-resident beam initialisation on a grid already in device memory
+ * (pypo_hip_generate_grid_dev); all bundle and frame arrays are device
+ * pointers; asynchronous on `stream` (NULL: the default stream) except
+ * pypo_hip_make_grt_frame_dev, which synchronises to check its acceptance
+ * flag.  Same formulas as the drop-in beam calls above. */
+int pypo_hip_make_gauss_dev(GPODict gdict, const reflcontainer *grid, c2Bundle *field,
+                            c2Bundle *current, void *stream);
+int pypo_hip_make_gauss_beam_dev(vecGPODict gdict, const reflcontainer *grid, c2Bundle *field,
+                                 c2Bundle *current, void *stream);
+int pypo_hip_make_scalar_gauss_dev(ScalarGPODict sgdict, const reflcontainer *grid, arrC1 *field,
+                                   void *stream);
+int pypo_hip_calc_currents_dev(const c2Bundle *field, c2Bundle *current, const reflcontainer *grid,
+                               int mode, void *stream);
+int pypo_hip_make_rt_frame_dev(RTDict rdict, cframe *fr, void *stream);
+int pypo_hip_make_grt_frame_dev(GRTDict grdict, cframe *fr, void *stream);
 
 /* Launch shape of the single-propagation PO sums (pypo_hip_propagate_dev and
  * the drop-in symbols): targets per lane of the far-field kernels (1 or 2)

@@ -204,3 +204,70 @@ def propagate_on_grids(lib, mode, src_rows, cells, tgt_rows, cur_rows, k, nthrea
       ctypes.byref(cs), ctypes.byref(ct), ctypes.byref(cur), float(k), int(nthreads), float(epsilon),
       -1.0 if exp == "fwd" else 1.0)
     return res._keep
+
+
+# ---- beam initialisation (InterfaceBeam.h:21-34) -------------------------------------
+
+def _beam_fn(lib, name, argtypes):
+    f = getattr(lib.cdll, lib.pre + name)
+    f.argtypes = argtypes
+    f.restype = None
+    return f
+
+
+def make_gauss(lib, gdict, source, vector=False):
+    """makeGauss (vector=False) or makeGaussBeam (vector=True) through the
+    checker library -> (fields, currents)."""
+    from pypo_amd import beam as PB
+    P = ctypes.POINTER
+    shape = gridsize(source)
+    n = int(np.prod(shape))
+    f, c = MU.alloc_bundle("c2Bundle", n), MU.alloc_bundle("c2Bundle", n)
+    if vector:
+        fn = _beam_fn(lib, "makeGaussBeam", [PS.vecGPODict, PS.reflparams, P(PS.c2Bundle), P(PS.c2Bundle)])
+        fn(PB.fill_vecGPODict(gdict), MU.fill_reflparams(source), ctypes.byref(f), ctypes.byref(c))
+    else:
+        fn = _beam_fn(lib, "makeGauss", [PS.GPODict, PS.reflparams, P(PS.c2Bundle), P(PS.c2Bundle)])
+        g = PB.fill_GPODict(gdict)
+        fn(g, MU.fill_reflparams(source), ctypes.byref(f), ctypes.byref(c))
+    return MU.c2bundle_to_obj(f, shape, "fields"), MU.c2bundle_to_obj(c, shape, "currents")
+
+
+def make_scalar_gauss(lib, gdict, source):
+    from pypo_amd import beam as PB
+    shape = gridsize(source)
+    f = MU.alloc_bundle("arrC1", int(np.prod(shape)))
+    fn = _beam_fn(lib, "makeScalarGauss", [PS.ScalarGPODict, PS.reflparams, ctypes.POINTER(PS.arrC1)])
+    fn(PB.fill_SGPODict(gdict), MU.fill_reflparams(source), ctypes.byref(f))
+    return MU.arrc1_to_obj(f, shape)
+
+
+def calc_currents(lib, fields, source, mode):
+    shape = gridsize(source)
+    n = int(np.prod(shape))
+    fin = MU.fill_c2bundle(fields, n)
+    c = MU.alloc_bundle("c2Bundle", n)
+    P = ctypes.POINTER
+    fn = _beam_fn(lib, "calcCurrents", [P(PS.c2Bundle), P(PS.c2Bundle), PS.reflparams, ctypes.c_int])
+    fn(ctypes.byref(fin), ctypes.byref(c), MU.fill_reflparams(source), int(mode))
+    return MU.c2bundle_to_obj(c, shape, "currents")
+
+
+def make_rt_frame(lib, rdict):
+    from pypo_amd import beam as PB
+    n = 1 + int(rdict["nRays"]) * 4 * int(rdict["nRing"])
+    res = MU.alloc_cframe(n)
+    fn = _beam_fn(lib, "makeRTframe", [PS.RTDict, ctypes.POINTER(PS.cframe)])
+    fn(PB.fill_RTDict(rdict), ctypes.byref(res))
+    return MU.cframe_to_frame(res)
+
+
+def make_grt_frame_ref(rdict):
+    """makeGRTframe of the compiled reference (random: it seeds from
+    std::random_device whatever the seed field says)."""
+    from pypo_amd import beam as PB
+    lib = load("ref")
+    res = MU.alloc_cframe(int(rdict["nRays"]))
+    fn = _beam_fn(lib, "makeGRTframe", [PS.GRTDict, ctypes.POINTER(PS.cframe)])
+    fn(PB.fill_GRTDict(rdict), ctypes.byref(res))
+    return MU.cframe_to_frame(res)

@@ -622,6 +622,181 @@ void trace(const reflparams &tp, cframe *in, cframe *out, int nthreads, double t
     transform_frame(tp.transf, out, false);
 }
 
+
+// ---- beam initialisation (src/common/BeamInit.h) ----------------------------------
+
+// Grid of the beam plane, as BeamInit.h generates it (transform on, spheric on).
+struct OwnedGrid {
+    std::vector<double> buf;
+    reflcontainer c;
+    explicit OwnedGrid(const reflparams &r) {
+        int n = r.n_cells[0] * r.n_cells[1];
+        buf.assign(7 * (size_t)n, 0.);
+        double *b = buf.data();
+        c = reflcontainer{n, b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, b + 6 * n};
+        make_grid(r, &c, true, true);
+    }
+};
+
+// n x v, real n, complex v.
+void ncross(const double *n, const cplx *v, cplx *out) {
+    out[0] = n[1] * v[2] - n[2] * v[1];
+    out[1] = n[2] * v[0] - n[0] * v[2];
+    out[2] = n[0] * v[1] - n[1] * v[0];
+}
+
+void store6(c2Bundle *b, int field, int i, const cplx *v, double f) {
+    double **p = (double **)b + 6 * field;
+    for (int a = 0; a < 3; a++) {
+        p[a][i] = f * v[a].real();
+        p[3 + a][i] = f * v[a].imag();
+    }
+}
+
+// Astigmatic Gaussian: initGauss, BeamInit.h:288-405 (with its E0-only
+// normalisation; the scalar twin below keeps sqrt(2 / (pi wzx wzy))).
+cplx astig_gauss(double lam, double w0x, double w0y, double nidx, double dxyz, double x, double y,
+                 double z) {
+    const cplx j(0., 1.);
+    double zRx = M_PI * w0x * w0x * nidx / lam, zRy = M_PI * w0y * w0y * nidx / lam;
+    double k = 2 * M_PI / lam;
+    double zy = z - dxyz;
+    double wzx = w0x * std::sqrt(1 + (z / zRx) * (z / zRx));
+    double wzy = w0y * std::sqrt(1 + (zy / zRy) * (zy / zRy));
+    double rx = z / (z * z + zRx * zRx), ry = zy / (zy * zy + zRy * zRy);
+    double phx = std::atan(z / zRx), phy = std::atan(zy / zRy);
+    return std::exp(-(x / wzx) * (x / wzx) - (y / wzy) * (y / wzy) -
+                    j * M_PI / lam * (x * x * rx + y * y * ry) - j * k * z + j * (phx - phy) * 0.5);
+}
+
+void make_gauss(const GPODict &g, const reflparams &plane, c2Bundle *field, c2Bundle *cur) {
+    OwnedGrid G(plane);
+    const reflcontainer &c = G.c;
+    const cplx zero3[3] = {0., 0., 0.};
+    for (int i = 0; i < c.size; i++) {
+        cplx f = g.E0 * astig_gauss(g.lam, g.w0x, g.w0y, g.n, g.dxyz, c.x[i], c.y[i], c.z[i]);
+        cplx e[3] = {f * g.pol[0], f * g.pol[1], f * g.pol[2]};
+        double nv[3] = {c.nx[i], c.ny[i], c.nz[i]};
+        cplx m[3];
+        ncross(nv, e, m);
+        store6(field, 0, i, e, 1.);
+        store6(field, 1, i, zero3, 1.);
+        store6(cur, 0, i, zero3, 1.);
+        store6(cur, 1, i, m, -2.);
+    }
+}
+
+void make_scalar_gauss(const ScalarGPODict &g, const reflparams &plane, arrC1 *out) {
+    OwnedGrid G(plane);
+    const reflcontainer &c = G.c;
+    for (int i = 0; i < c.size; i++) {
+        double z = c.z[i], zy = z - g.dxyz;
+        double zRx = M_PI * g.w0x * g.w0x * g.n / g.lam, zRy = M_PI * g.w0y * g.w0y * g.n / g.lam;
+        double wzx = g.w0x * std::sqrt(1 + (z / zRx) * (z / zRx));
+        double wzy = g.w0y * std::sqrt(1 + (zy / zRy) * (zy / zRy));
+        cplx f = g.E0 * std::sqrt(2 / (M_PI * wzx * wzy)) *
+                 astig_gauss(g.lam, g.w0x, g.w0y, g.n, g.dxyz, c.x[i], c.y[i], z);
+        out->x[i] = f.real();
+        out->y[i] = f.imag();
+    }
+}
+
+// Complex-source-point vector Gaussian: initGaussBeam, BeamInit.h:407-596.
+void make_gauss_beam(const vecGPODict &g, const reflparams &plane, c2Bundle *field, c2Bundle *cur) {
+    OwnedGrid G(plane);
+    const reflcontainer &c = G.c;
+    const cplx j(0., 1.);
+    const cplx zero3[3] = {0., 0., 0.};
+    double k = 2 * M_PI / g.lam;
+    double zc = k * g.w0 * g.w0 / 2.0;
+    double pref = g.w0 * k * std::sqrt(g.power / (8 * M_PI));
+    double F = (g.mode == 1 || g.mode == 2) ? 2. : 1.;
+    for (int i = 0; i < c.size; i++) {
+        // BeamInit.h:473 calls abs() on a double where only the integer abs is
+        // visible: the axial distance is |trunc(z + z0)|, not |z + z0|.
+        cplx R[3] = {cplx(c.x[i], 0.), cplx(c.y[i], 0.),
+                     cplx(std::fabs(std::trunc(c.z[i] + g.z)), 0.) + j * zc};
+        cplx r = std::sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+        cplx ex = std::exp(-j * k * r - k * k * g.w0 * g.w0 / 2.0);
+        cplx q = 1.0 / (k * r);
+        cplx s1 = -j * q - q * q + j * q * q * q;
+        cplx s2 = j * q + 3.0 * q * q - 3.0 * j * q * q * q;
+        cplx s3 = j * q + q * q;
+        cplx e[3], h[3];
+        e[0] = pref * ex * (s1 + s2 * (R[0] * R[0] / (r * r)) - s3 * (R[2] / r));
+        e[1] = pref * ex * s2 * R[0] * R[1] / (r * r);
+        e[2] = pref * ex * (s2 * R[0] * R[2] / (r * r) - s3 * R[0] / r);
+        h[0] = pref * ex * s2 * R[0] * R[1] / (r * r);
+        h[1] = pref * ex * (s1 + s2 * (R[1] * R[1] / (r * r)) - s3 * (R[2] / r));
+        h[2] = pref * ex * (s2 * R[1] * R[2] / (r * r) - s3 * R[1] / r);
+        if (!(c.z[i] + g.z >= 0.0))
+            for (int a = 0; a < 3; a++) { e[a] = std::conj(e[a]); h[a] = std::conj(h[a]); }
+        double nv[3] = {c.nx[i], c.ny[i], c.nz[i]};
+        cplx M[3], J[3];
+        ncross(nv, e, M);
+        ncross(nv, h, J);
+        store6(field, 0, i, e, 1.);
+        store6(field, 1, i, h, 1.);
+        if (g.mode == 0 || g.mode == 2) store6(cur, 0, i, J, F);
+        else store6(cur, 0, i, zero3, 1.);
+        if (g.mode == 0 || g.mode == 1) store6(cur, 1, i, M, -F);
+        else store6(cur, 1, i, zero3, 1.);
+    }
+}
+
+// calcJM, BeamInit.h:660-844.
+void calc_currents(const c2Bundle *field, c2Bundle *cur, const reflparams &plane, int mode) {
+    OwnedGrid G(plane);
+    const reflcontainer &c = G.c;
+    const cplx zero3[3] = {0., 0., 0.};
+    double *const *f = (double *const *)field;
+    for (int i = 0; i < c.size; i++) {
+        cplx e[3], h[3];
+        for (int a = 0; a < 3; a++) {
+            e[a] = cplx(f[a][i], f[3 + a][i]);
+            h[a] = cplx(f[6 + a][i], f[9 + a][i]);
+        }
+        double nv[3] = {c.nx[i], c.ny[i], c.nz[i]};
+        cplx ne[3], nh[3];
+        ncross(nv, e, ne);
+        ncross(nv, h, nh);
+        if (mode == 0) { store6(cur, 1, i, ne, -1.); store6(cur, 0, i, nh, 1.); }
+        else if (mode == 1) { store6(cur, 1, i, ne, -2.); store6(cur, 0, i, zero3, 1.); }
+        else if (mode == 2) { store6(cur, 0, i, nh, 2.); store6(cur, 1, i, zero3, 1.); }
+        else if (mode == 3) { store6(cur, 0, i, ne, 2.); store6(cur, 1, i, zero3, 1.); }
+    }
+}
+
+// Tubular frame: initFrame, BeamInit.h:159-214 (same running angle sum).
+void make_rt_frame(const RTDict &r, cframe *fr) {
+    int nTot = 1 + r.nRing * 4 * r.nRays;
+    fr->size = nTot;
+    fr->x[0] = fr->y[0] = fr->z[0] = 0.;
+    fr->dx[0] = fr->dy[0] = 0.;
+    fr->dz[0] = 1.;
+    double alpha = 0., d_alpha = r.nRays > 0 ? 2 * M_PI / (4 * r.nRays) : 0.;
+    int ring = 1;
+    for (int i = 1; i < nTot; i++) {
+        fr->x[i] = r.x0 * std::cos(alpha) / r.nRing * ring;
+        fr->y[i] = r.y0 * std::sin(alpha) / r.nRing * ring;
+        fr->z[i] = 0.;
+        double rx = r.angy0 * std::sin(alpha) / r.nRing * ring;
+        double ry = r.angx0 * std::cos(alpha) / r.nRing * ring;
+        double rz = 2 * alpha;
+        // (0, 0, 1) rotated by Utils::matRot's Euler angles (Utils.h:613-635)
+        double cx = std::cos(rx), cy = std::cos(ry), cz = std::cos(rz);
+        double sx = std::sin(rx), sy = std::sin(ry), sz = std::sin(rz);
+        fr->dx[i] = cz * sy * cx + sz * sx;
+        fr->dy[i] = sz * sy * cx - cz * sx;
+        fr->dz[i] = cy * cx;
+        alpha += d_alpha;
+        if (i == (nTot / r.nRing) * ring) {
+            ring++;
+            alpha = 0.;
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -711,5 +886,19 @@ void oracle_farfield_subset(c2Bundle *res, reflparams s, reflcontainer *cs, refl
         }
     });
 }
+
+void oracle_makeGauss(GPODict g, reflparams plane, c2Bundle *field, c2Bundle *cur) {
+    make_gauss(g, plane, field, cur);
+}
+void oracle_makeGaussBeam(vecGPODict g, reflparams plane, c2Bundle *field, c2Bundle *cur) {
+    make_gauss_beam(g, plane, field, cur);
+}
+void oracle_makeScalarGauss(ScalarGPODict g, reflparams plane, arrC1 *field) {
+    make_scalar_gauss(g, plane, field);
+}
+void oracle_calcCurrents(c2Bundle *field, c2Bundle *cur, reflparams plane, int mode) {
+    calc_currents(field, cur, plane, mode);
+}
+void oracle_makeRTframe(RTDict r, cframe *fr) { make_rt_frame(r, fr); }
 
 }  // extern "C"

@@ -264,6 +264,116 @@ int po_dropin(int mode, void *res, const reflparams &sp, const reflparams &tp, r
     return 0;
 }
 
+// ---- beam initialisation into host buffers ------------------------------------------
+enum BeamKind { BEAM_GAUSS, BEAM_VEC_GAUSS, BEAM_SCALAR, BEAM_CURRENTS };
+
+// Generate `plane`'s grid on the device, run one beam kernel, copy the
+// outputs (12 field + 12 current arrays, or 2 scalar arrays, or 12 current
+// arrays) into the host arrays.  h_in: the 12 field arrays of calcCurrents.
+int beam_dropin(int kind, const void *params, const reflparams &plane, double *const *h_out, int n_out,
+                double *const *h_in, int mode) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    PYPO_REQUIRE(plane.n_cells, ERR_ARG, "beam: reflparams has a NULL n_cells");
+    const long long nn = (long long)plane.n_cells[0] * plane.n_cells[1];
+    PYPO_REQUIRE(nn > 0 && nn < (1LL << 31), ERR_ARG, "beam: grid size out of range");
+    const int n = (int)nn;
+    if ((err = check_refl(plane, n, "beam"))) return err;
+    for (int q = 0; q < n_out; q++) PYPO_REQUIRE(h_out[q], ERR_ARG, "beam: result bundle has a NULL array");
+    if (h_in)
+        for (int q = 0; q < 12; q++) PYPO_REQUIRE(h_in[q], ERR_ARG, "beam: field bundle has a NULL array");
+    hipStream_t st = g_stream;
+    if ((err = begin_call(st))) return err;
+    double *dg = (double *)pool_get(SLOT_SRC_GRID, 7 * (size_t)n * sizeof(double), &err);
+    if (err) return err;
+    GridPtrs g = carve_grid(dg, n);
+    if ((err = device_grid(plane, g, n, SLOT_ERRS_S, true, true, st))) return err;
+    double *dout = (double *)pool_get(SLOT_RESULT, (size_t)n_out * n * sizeof(double), &err);
+    if (err) return err;
+    Bundle12 o1{}, o2{};
+    for (int q = 0; q < 12 && q < n_out; q++) o1.p[q] = dout + (size_t)q * n;
+    for (int q = 12; q < n_out; q++) o2.p[q - 12] = dout + (size_t)q * n;
+    switch (kind) {
+        case BEAM_GAUSS: err = launch_gauss(*(const GPODict *)params, g, n, o1, o2, st); break;
+        case BEAM_VEC_GAUSS: err = launch_gauss_beam(*(const vecGPODict *)params, g, n, o1, o2, st); break;
+        case BEAM_SCALAR:
+            err = launch_scalar_gauss(*(const ScalarGPODict *)params, g, n, o1.p[0], o1.p[1], st);
+            break;
+        default: {
+            double *din = (double *)pool_get(SLOT_CURRENTS, 12 * (size_t)n * sizeof(double), &err);
+            if (err) return err;
+            Bundle12 fin{};
+            for (int q = 0; q < 12; q++) {
+                fin.p[q] = din + (size_t)q * n;
+                PYPO_HIP_TRY(hipMemcpyAsync(fin.p[q], h_in[q], n * sizeof(double), hipMemcpyHostToDevice, st));
+            }
+            err = launch_calc_jm(fin, g, n, mode, o1, st);
+        }
+    }
+    if (err) return err;
+    for (int q = 0; q < n_out; q++)
+        PYPO_HIP_TRY(hipMemcpyAsync(h_out[q], dout + (size_t)q * n, n * sizeof(double),
+                                    hipMemcpyDeviceToHost, st));
+    if ((err = end_call(st))) return err;
+    PYPO_HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+uint64_t grt_key(int seed) {
+    if (seed != -1) return 0x9E3779B97F4A7C15ull ^ (uint64_t)(uint32_t)seed;
+    std::random_device rd;
+    return ((uint64_t)rd() << 32) | rd();
+}
+
+// Ray frames into device arrays (the frame struct holds device pointers).
+int frame_dev(bool gauss, const void *params, const cframe &fr, hipStream_t st) {
+    if (!gauss) return launch_rt_frame(*(const RTDict *)params, fr, st);
+    const GRTDict &g = *(const GRTDict *)params;
+    int err = 0;
+    int *flag = (int *)pool_get(SLOT_FLAG, 256, &err);
+    if (err) return err;
+    PYPO_HIP_TRY(hipMemsetAsync(flag, 0, sizeof(int), st));
+    if ((err = launch_grt_frame(g, grt_key(g.seed), fr, flag, kGrtMaxTrials, st))) return err;
+    int h = 0;
+    PYPO_HIP_TRY(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    PYPO_HIP_TRY(hipStreamSynchronize(st));
+    PYPO_REQUIRE(h == 0, ERR_ARG,
+                 "makeGRTframe: some rays found no accepted candidate within 2^20 trials "
+                 "(the Gaussian pdf peak 1/(pi^2 x0 y0 angx0 angy0) is too small)");
+    return 0;
+}
+
+int frame_dropin(bool gauss, const void *params, int n, cframe *fr) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    PYPO_REQUIRE(fr && fr->x && fr->y && fr->z && fr->dx && fr->dy && fr->dz, ERR_ARG,
+                 "ray frame: NULL cframe or array");
+    PYPO_REQUIRE(n >= 1, ERR_ARG, "ray frame: no rays");
+    hipStream_t st = g_stream;
+    if ((err = begin_call(st))) return err;
+    double *d = (double *)pool_get(SLOT_RAYS, 6 * (size_t)n * sizeof(double), &err);
+    if (err) return err;
+    cframe df{n, d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n, d + 4 * (size_t)n, d + 5 * (size_t)n};
+    if ((err = frame_dev(gauss, params, df, st))) return err;
+    double *hs[6] = {fr->x, fr->y, fr->z, fr->dx, fr->dy, fr->dz};
+    for (int q = 0; q < 6; q++)
+        PYPO_HIP_TRY(hipMemcpyAsync(hs[q], d + (size_t)q * n, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    if ((err = end_call(st))) return err;
+    PYPO_HIP_TRY(hipStreamSynchronize(st));
+    fr->size = n;
+    return 0;
+}
+
+int check_bundle12(const c2Bundle *b, const char *what) {
+    if (!b) { set_error(ERR_ARG, std::string(what) + ": NULL bundle"); return ERR_ARG; }
+    double *const *p = (double *const *)b;
+    for (int q = 0; q < 12; q++)
+        if (!p[q]) { set_error(ERR_ARG, std::string(what) + ": bundle has a NULL array"); return ERR_ARG; }
+    return 0;
+}
+
 }  // namespace
 }  // namespace pypo
 
@@ -577,6 +687,141 @@ int pypo_hip_sweep_destroy(void *handle) {
 int pypo_hip_set_sum_shape(int tpl, int want_wgs) {
     std::lock_guard<std::recursive_mutex> lock(g_mu);
     return set_sum_shape(tpl, want_wgs);
+}
+
+// ---- beam initialisation (InterfaceBeam.h:21-34) -----------------------------------
+
+void makeGauss(GPODict gdict, reflparams plane, c2Bundle *res_field, c2Bundle *res_current) {
+    if (check_bundle12(res_field, "makeGauss field") || check_bundle12(res_current, "makeGauss current"))
+        return;
+    double *out[24];
+    for (int q = 0; q < 12; q++) {
+        out[q] = ((double **)res_field)[q];
+        out[12 + q] = ((double **)res_current)[q];
+    }
+    beam_dropin(BEAM_GAUSS, &gdict, plane, out, 24, nullptr, 0);
+}
+
+void makeGaussBeam(vecGPODict gdict, reflparams plane, c2Bundle *res_field, c2Bundle *res_current) {
+    if (check_bundle12(res_field, "makeGaussBeam field") ||
+        check_bundle12(res_current, "makeGaussBeam current"))
+        return;
+    double *out[24];
+    for (int q = 0; q < 12; q++) {
+        out[q] = ((double **)res_field)[q];
+        out[12 + q] = ((double **)res_current)[q];
+    }
+    beam_dropin(BEAM_VEC_GAUSS, &gdict, plane, out, 24, nullptr, 0);
+}
+
+void makeScalarGauss(ScalarGPODict sgdict, reflparams plane, arrC1 *res_field) {
+    if (!res_field) {
+        set_error(ERR_ARG, "makeScalarGauss: NULL result");
+        return;
+    }
+    double *out[2] = {res_field->x, res_field->y};
+    beam_dropin(BEAM_SCALAR, &sgdict, plane, out, 2, nullptr, 0);
+}
+
+void calcCurrents(c2Bundle *res_field, c2Bundle *res_current, reflparams rdict, int mode) {
+    if (check_bundle12(res_field, "calcCurrents field") ||
+        check_bundle12(res_current, "calcCurrents current"))
+        return;
+    beam_dropin(BEAM_CURRENTS, nullptr, rdict, (double *const *)res_current, 12,
+                (double *const *)res_field, mode);
+}
+
+void makeRTframe(RTDict rdict, cframe *fr) {
+    if (rdict.nRays < 0 || rdict.nRing < 0) {
+        set_error(ERR_ARG, "makeRTframe: nRays and nRing must be >= 0");
+        return;
+    }
+    frame_dropin(false, &rdict, rt_frame_size(rdict), fr);
+}
+
+void makeGRTframe(GRTDict grdict, cframe *fr) { frame_dropin(true, &grdict, grdict.nRays, fr); }
+
+int pypo_hip_make_gauss_dev(GPODict gdict, const reflcontainer *grid, c2Bundle *field,
+                            c2Bundle *current, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    if ((err = check_container(grid, "grid")) || (err = check_bundle12(field, "field")) ||
+        (err = check_bundle12(current, "current")))
+        return err;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the default stream
+    if ((err = begin_call(st))) return err;
+    if ((err = launch_gauss(gdict, grid_ptrs(*grid), grid->size, bundle12(*field), bundle12(*current), st)))
+        return err;
+    return end_call(st);
+}
+
+int pypo_hip_make_gauss_beam_dev(vecGPODict gdict, const reflcontainer *grid, c2Bundle *field,
+                                 c2Bundle *current, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    if ((err = check_container(grid, "grid")) || (err = check_bundle12(field, "field")) ||
+        (err = check_bundle12(current, "current")))
+        return err;
+    hipStream_t st = (hipStream_t)stream;
+    if ((err = begin_call(st))) return err;
+    if ((err = launch_gauss_beam(gdict, grid_ptrs(*grid), grid->size, bundle12(*field), bundle12(*current),
+                                 st)))
+        return err;
+    return end_call(st);
+}
+
+int pypo_hip_make_scalar_gauss_dev(ScalarGPODict sgdict, const reflcontainer *grid, arrC1 *field,
+                                   void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    if ((err = check_container(grid, "grid"))) return err;
+    PYPO_REQUIRE(field && field->x && field->y, ERR_ARG, "field: NULL arrC1 or array");
+    hipStream_t st = (hipStream_t)stream;
+    if ((err = begin_call(st))) return err;
+    if ((err = launch_scalar_gauss(sgdict, grid_ptrs(*grid), grid->size, field->x, field->y, st))) return err;
+    return end_call(st);
+}
+
+int pypo_hip_calc_currents_dev(const c2Bundle *field, c2Bundle *current, const reflcontainer *grid,
+                               int mode, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    if ((err = check_container(grid, "grid")) || (err = check_bundle12(field, "field")) ||
+        (err = check_bundle12(current, "current")))
+        return err;
+    hipStream_t st = (hipStream_t)stream;
+    if ((err = begin_call(st))) return err;
+    if ((err = launch_calc_jm(bundle12(*field), grid_ptrs(*grid), grid->size, mode, bundle12(*current), st)))
+        return err;
+    return end_call(st);
+}
+
+int pypo_hip_make_rt_frame_dev(RTDict rdict, cframe *fr, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    PYPO_REQUIRE(fr && fr->x && fr->y && fr->z && fr->dx && fr->dy && fr->dz, ERR_ARG,
+                 "ray frame: NULL cframe or array");
+    hipStream_t st = (hipStream_t)stream;
+    if ((err = begin_call(st))) return err;
+    if ((err = frame_dev(false, &rdict, *fr, st))) return err;
+    return end_call(st);
+}
+
+int pypo_hip_make_grt_frame_dev(GRTDict grdict, cframe *fr, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    PYPO_REQUIRE(fr && fr->x && fr->y && fr->z && fr->dx && fr->dy && fr->dz, ERR_ARG,
+                 "ray frame: NULL cframe or array");
+    hipStream_t st = (hipStream_t)stream;
+    if ((err = begin_call(st))) return err;
+    if ((err = frame_dev(true, &grdict, *fr, st))) return err;
+    return end_call(st);
 }
 
 // ---- adjoint of the far-field power -------------------------------------------------

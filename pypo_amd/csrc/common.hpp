@@ -165,7 +165,7 @@ constexpr double kTwoPi = 6.283185307179586476925286766559;
 void *pool_get(int slot, size_t bytes, int *err);
 enum PoolSlot {
     SLOT_SRC_GRID = 0, SLOT_TGT_GRID, SLOT_CURRENTS, SLOT_RESULT, SLOT_PACKED,
-    SLOT_PARTIAL, SLOT_ERRS_S, SLOT_ERRS_T, SLOT_RAYS, SLOT_RT_ERRS, SLOT_ADJ,
+    SLOT_PARTIAL, SLOT_ERRS_S, SLOT_ERRS_T, SLOT_RAYS, SLOT_RT_ERRS, SLOT_ADJ, SLOT_FLAG,
     SLOT_COUNT
 };
 
@@ -234,6 +234,21 @@ int launch_ff_power_grad(int precision, const GridPtrs &dish, int n_dish, int gm
                          int fncy, const Bundle12 &feed_cur, const GridPtrs &ff, int n_tgt,
                          const Bundle12 &E, const double *g, double k, double tdir, double *grad,
                          void *ws, size_t ws_bytes, hipStream_t st);
+
+// Beam initialisation (beam.hip).  Grids and bundles are device memory.
+int launch_gauss(const GPODict &g, const GridPtrs &gr, int n, const Bundle12 &field, const Bundle12 &cur,
+                 hipStream_t st);
+int launch_scalar_gauss(const ScalarGPODict &g, const GridPtrs &gr, int n, double *ur, double *ui,
+                        hipStream_t st);
+int launch_gauss_beam(const vecGPODict &g, const GridPtrs &gr, int n, const Bundle12 &field,
+                      const Bundle12 &cur, hipStream_t st);
+int launch_calc_jm(const Bundle12 &field, const GridPtrs &gr, int n, int mode, const Bundle12 &cur,
+                   hipStream_t st);
+int rt_frame_size(const RTDict &r);
+int launch_rt_frame(const RTDict &r, const cframe &fr, hipStream_t st);
+int launch_grt_frame(const GRTDict &g, uint64_t seed, const cframe &fr, int *d_fail, int max_trials,
+                     hipStream_t st);
+constexpr int kGrtMaxTrials = 1 << 20;
 
 // Kernel timing of the last timed launch (HIP events on the launch stream).
 void record_kernel_ms(double ms);

@@ -67,3 +67,34 @@ class cframe(ctypes.Structure):
 
 
 BUNDLE_FIELDS = {"c2Bundle": _C2, "c4Bundle": _C4, "c2rBundle": _C2R, "arrC1": ["x", "y"]}
+
+
+class RTDict(ctypes.Structure):
+    """Tubular ray-frame parameters, angles in radians (Structs.h:164-171)."""
+    _fields_ = [("nRays", ctypes.c_int), ("nRing", ctypes.c_int), ("angx0", ctypes.c_double),
+                ("angy0", ctypes.c_double), ("x0", ctypes.c_double), ("y0", ctypes.c_double)]
+
+
+class GRTDict(ctypes.Structure):
+    """Gaussian ray-frame parameters, angles in radians (Structs.h:174-181)."""
+    _fields_ = [("nRays", ctypes.c_int), ("angx0", ctypes.c_double), ("angy0", ctypes.c_double),
+                ("x0", ctypes.c_double), ("y0", ctypes.c_double), ("seed", ctypes.c_int)]
+
+
+class GPODict(ctypes.Structure):
+    """Astigmatic Gaussian beam parameters (Structs.h:184-192)."""
+    _fields_ = [("lam", ctypes.c_double), ("w0x", ctypes.c_double), ("w0y", ctypes.c_double),
+                ("n", ctypes.c_double), ("E0", ctypes.c_double), ("dxyz", ctypes.c_double),
+                ("pol", _dp)]
+
+
+class vecGPODict(ctypes.Structure):
+    """Complex-source-point vector Gaussian beam parameters (Structs.h:195-202)."""
+    _fields_ = [("lam", ctypes.c_double), ("w0", ctypes.c_double), ("z", ctypes.c_double),
+                ("n", ctypes.c_double), ("power", ctypes.c_double), ("mode", ctypes.c_int)]
+
+
+class ScalarGPODict(ctypes.Structure):
+    """Scalar Gaussian beam parameters (Structs.h:206-213)."""
+    _fields_ = [("lam", ctypes.c_double), ("w0x", ctypes.c_double), ("w0y", ctypes.c_double),
+                ("n", ctypes.c_double), ("E0", ctypes.c_double), ("dxyz", ctypes.c_double)]

@@ -140,3 +140,61 @@ def random_scalar(shape, seed):
     from pypo_amd import types as T
     rng = np.random.default_rng(seed)
     return T.scalarfield(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+
+
+def _beam_plane(n=(15, 13), z=0.0, tilt=False, half=(4.0, 3.0)):
+    p = G.plane({"gmode": "xy", "lims_x": [-half[0], half[0]], "lims_y": [-half[1], half[1]],
+                 "gridsize": list(n)})
+    m = np.eye(4)
+    if tilt:
+        m = G.find_rotation([0, 0, 1], [0.2, -0.1, 1.0])
+    p["transf"] = G.mat_translate([0.3, -0.2, z], m)
+    return p
+
+
+def beam_cases():
+    """name -> dict(kind, params, source[, mode, seed]) for the beam-initialisation calls."""
+    cases = {
+        "gauss_waist": {"kind": "gauss", "source": _beam_plane(),
+                        "params": {"lam": 1.0, "w0x": 2.0, "w0y": 2.0, "n": 1.0, "E0": 1.0, "dxyz": 0.0,
+                                   "pol": np.array([1.0, 0.0, 0.0])}},
+        "gauss_astig": {"kind": "gauss", "source": _beam_plane(z=7.0, tilt=True),
+                        "params": {"lam": 0.8, "w0x": 1.5, "w0y": 2.5, "n": 1.3, "E0": 2.0, "dxyz": 5.0,
+                                   "pol": np.array([0.6, 0.8, 0.0])}},
+        "scalar_astig": {"kind": "scalar", "source": _beam_plane(z=-4.0, tilt=True),
+                         "params": {"lam": 1.2, "w0x": 1.5, "w0y": 2.0, "n": 1.1, "E0": 3.0, "dxyz": 2.0}},
+        "scalar_waist": {"kind": "scalar", "source": _beam_plane(),
+                         "params": {"lam": 1.0, "w0x": 2.0, "w0y": 1.0, "n": 1.0, "E0": 1.0, "dxyz": 0.0}},
+    }
+    for mode in (0, 1, 2):
+        cases[f"vec_gauss_m{mode}"] = {
+            "kind": "vec", "source": _beam_plane(z=0.0),
+            "params": {"lam": 1.0, "w0": 1.5, "z": 0.0, "n": 1.0, "power": 4 * np.pi, "mode": mode}}
+    # z + gdict.z < 0 on part of the tilted plane: the conjugated branch
+    cases["vec_gauss_behind"] = {
+        "kind": "vec", "source": _beam_plane(z=-1.0, tilt=True, half=(6.0, 5.0)),
+        "params": {"lam": 0.9, "w0": 2.0, "z": 0.5, "n": 1.0, "power": 2.0, "mode": 0}}
+    for mode in (0, 1, 2, 3):
+        cases[f"currents_m{mode}"] = {"kind": "currents", "source": _beam_plane(z=2.0, tilt=True),
+                                      "mode": mode, "seed": 40 + mode}
+    cases["rt_tube_3ring"] = {"kind": "rt", "params": {"nRays": 4, "nRing": 3, "angx0": 2.0, "angy0": 3.0,
+                                                       "x0": 5.0, "y0": 4.0}}
+    cases["rt_tube_1ring"] = {"kind": "rt", "params": {"nRays": 7, "nRing": 1, "angx0": 1.0, "angy0": 0.5,
+                                                       "x0": 2.0, "y0": 2.0}}
+    cases["rt_tube_chief"] = {"kind": "rt", "params": {"nRays": 0, "nRing": 2, "angx0": 1.0, "angy0": 1.0,
+                                                       "x0": 1.0, "y0": 1.0}}
+    return cases
+
+
+def grt_case():
+    """Gaussian ray frame: pdf peak 1/(pi^2 x0 y0 angx0 angy0) = 0.405 < 1, so
+    the accepted rays follow the truncated Gaussian undistorted."""
+    return {"nRays": 1000, "x0": 5.0, "y0": 5.0, "angx0": float(np.degrees(0.1)),
+            "angy0": float(np.degrees(0.1)), "seed": 7}
+
+
+def random_fields(shape, seed):
+    from pypo_amd import types as T
+    rng = np.random.default_rng(seed)
+    comps = [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(6)]
+    return T.fields(*comps)

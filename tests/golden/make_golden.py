@@ -1,9 +1,11 @@
 """Write the golden vectors: run every case of cases.py through the reference's
 own CPU library, oracle/_ref/libpypocpu_ref.so, compiled from the reference's
-C++ sources (src/CPU/InterfaceCPU.cpp, src/common/InterfaceReflector.cpp) by
-`make -C oracle ref`.  Outputs: po.npz, grids.npz, rt.npz in this directory.
+C++ sources (src/CPU/InterfaceCPU.cpp, src/common/InterfaceReflector.cpp,
+src/common/InterfaceBeam.cpp) by `make -C oracle ref`.  Outputs: po.npz,
+grids.npz, rt.npz, beams.npz in this directory.
 
-usage: python tests/golden/make_golden.py
+usage: python tests/golden/make_golden.py          (po, grids, rt, beams)
+       python tests/golden/make_golden.py beams    (beams.npz only)
 """
 
 import os
@@ -31,8 +33,35 @@ def flat(o):
     return np.stack([np.ravel(o.x), np.ravel(o.y), np.ravel(o.z)]).astype(complex)
 
 
+def beams(ref):
+    """beams.npz: every beam case through the compiled reference's
+    src/common/InterfaceBeam.cpp, plus one (random) Gaussian ray frame."""
+    out = {}
+    for name, c in C.beam_cases().items():
+        if c["kind"] in ("gauss", "vec"):
+            f, cur = OB.make_gauss(ref, c["params"], c["source"], vector=c["kind"] == "vec")
+            out[f"{name}/field"] = flat(f)
+            out[f"{name}/current"] = flat(cur)
+        elif c["kind"] == "scalar":
+            out[f"{name}/field"] = np.ravel(OB.make_scalar_gauss(ref, c["params"], c["source"]).S)
+        elif c["kind"] == "currents":
+            fields = C.random_fields(gridsize(c["source"]), c["seed"])
+            out[f"{name}/field"] = flat(fields)
+            out[f"{name}/current"] = flat(OB.calc_currents(ref, fields, c["source"], c["mode"]))
+        else:
+            fr = OB.make_rt_frame(ref, c["params"])
+            out[f"{name}/frame"] = np.stack([getattr(fr, a) for a in ("x", "y", "z", "dx", "dy", "dz")])
+    fr = OB.make_grt_frame_ref(C.grt_case())
+    out["grt/frame"] = np.stack([getattr(fr, a) for a in ("x", "y", "z", "dx", "dy", "dz")])
+    np.savez_compressed(os.path.join(HERE, "beams.npz"), **out)
+    print("beams.npz", os.path.getsize(os.path.join(HERE, "beams.npz")), "bytes")
+
+
 def main():
     ref = OB.load("ref")
+    if sys.argv[1:] == ["beams"]:
+        beams(ref)
+        return
     po = {}
     for name, c in C.po_cases().items():
         shape = gridsize(c["source"])
@@ -63,6 +92,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "rt.npz"), **rt)
     for f in ("po.npz", "grids.npz", "rt.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
+    beams(ref)
 
 
 if __name__ == "__main__":
