@@ -266,11 +266,20 @@ def cpu_baseline(dish, ff, cur_dish, sub_rows):
     dt = time.perf_counter() - t
     vals = np.stack([c.ravel() for c in out.components()])
     pairs = N_DISH * N_DISH * nrow * N_FF
+    # linearity of the sampled rate: the same call on half the rows
+    half = G.plane({"gmode": "AoE", "lims_Az": [-FF_LIM, FF_LIM], "lims_El": [-FF_LIM, FF_LIM],
+                    "gridsize": [nrow // 2, N_FF]})
+    t = time.perf_counter()
+    OB.propagate(lib, "FF", dish, half, cur_dish, K, cores)
+    dt_half = time.perf_counter() - t
+    rate_half = N_DISH * N_DISH * (nrow // 2) * N_FF / dt_half
     return vals, {"value": pairs / dt, "unit": "pairs/s", "cores": cores,
                   "kind": "reference" if kind == "ref" else "port",
                   "sample": f"FF rows {sub_rows[0]}..{sub_rows[-1]} step {CPU_ROW_STRIDE} of the 501x501 "
                             f"far field ({nrow * N_FF} targets x {N_DISH * N_DISH} sources = {pairs:.3e} "
-                            f"pairs), one native call incl. its grid generation, {dt:.1f} s"}
+                            f"pairs), one native call incl. its grid generation, {dt:.1f} s",
+                  "linearity": {"rows": [nrow // 2, nrow], "pairs_per_s": [rate_half, pairs / dt],
+                                "ratio": round((pairs / dt) / rate_half, 3)}}
 
 
 def main():

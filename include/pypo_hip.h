@@ -307,6 +307,24 @@ int pypo_hip_calc_currents_dev(const c2Bundle *field, c2Bundle *current, const r
 int pypo_hip_make_rt_frame_dev(RTDict rdict, cframe *fr, void *stream);
 int pypo_hip_make_grt_frame_dev(GRTDict grdict, cframe *fr, void *stream);
 
+/* Hybrid ray-trace / PO step (System.runHybridPropagation, System.py:1570-1622)
+ * on device arrays: trace fr_in onto `target` into fr_out (as
+ * pypo_hip_trace_dev with one surface), then carry the field samples along
+ * the rays: field_out = field_in * exp(j k L1), times sqrt(L0 / (2 L1 + L0))
+ * when `start` (3 doubles, host memory) is not NULL; L1 = |p_out - p_in|,
+ * L0 = |p_in - start| (calcRayLen, System.py:1304-1372).  Asynchronous. */
+int pypo_hip_hybrid_dev(reflparams target, const cframe *fr_in, cframe *fr_out,
+                        const c2Bundle *field_in, c2Bundle *field_out, const double *start,
+                        double k, double tol, double t0, void *stream);
+
+/* Nearest-neighbour interpolation of the 12 arrays of `values`, defined on
+ * the positions of `points`, onto the positions of `grid` (interpFrame with
+ * method "nearest", System.py:1624-1692, which uses scipy's griddata):
+ * out[q] = values[the point nearest to grid point q]; ties go to the smallest
+ * point index.  Device arrays; asynchronous. */
+int pypo_hip_interp_nearest_dev(const cframe *points, const c2Bundle *values,
+                                const reflcontainer *grid, c2Bundle *out, void *stream);
+
 /* Launch shape of the single-propagation PO sums (pypo_hip_propagate_dev and
  * the drop-in symbols): targets per lane of the far-field kernels (1 or 2)
  * and the workgroup count the source splits aim for.  0 restores the

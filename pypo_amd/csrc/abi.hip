@@ -824,6 +824,57 @@ int pypo_hip_make_grt_frame_dev(GRTDict grdict, cframe *fr, void *stream) {
     return end_call(st);
 }
 
+// ---- hybrid ray-trace / PO (System.runHybridPropagation) -----------------------------
+
+static int check_frame(const cframe *f, const char *what) {
+    if (!f || !f->x || !f->y || !f->z || !f->dx || !f->dy || !f->dz) {
+        set_error(ERR_ARG, std::string(what) + ": NULL cframe or array");
+        return ERR_ARG;
+    }
+    return 0;
+}
+
+int pypo_hip_hybrid_dev(reflparams target, const cframe *fr_in, cframe *fr_out, const c2Bundle *field_in,
+                        c2Bundle *field_out, const double *start, double k, double tol, double t0,
+                        void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    if ((err = check_frame(fr_in, "fr_in")) || (err = check_frame(fr_out, "fr_out")) ||
+        (err = check_bundle12(field_in, "field_in")) || (err = check_bundle12(field_out, "field_out")))
+        return err;
+    PYPO_REQUIRE(fr_in->size > 0 && fr_out->size == fr_in->size, ERR_ARG,
+                 "hybrid: fr_in and fr_out must have the same non-zero size");
+    cframe frames[2] = {*fr_in, *fr_out};
+    if ((err = pypo_hip_trace_dev(&target, 1, frames, tol, t0, stream))) return err;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the default stream
+    if ((err = begin_call(st))) return err;
+    if ((err = launch_hybrid_phase(*fr_in, *fr_out, bundle12(*field_in), bundle12(*field_out), fr_in->size,
+                                   k, start, st)))
+        return err;
+    return end_call(st);
+}
+
+int pypo_hip_interp_nearest_dev(const cframe *points, const c2Bundle *values, const reflcontainer *grid,
+                                c2Bundle *out, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    if ((err = check_frame(points, "points")) || (err = check_bundle12(values, "values")) ||
+        (err = check_container(grid, "grid")) || (err = check_bundle12(out, "out")))
+        return err;
+    PYPO_REQUIRE(points->size > 0 && grid->size > 0, ERR_ARG, "interpolation: empty point set or grid");
+    hipStream_t st = (hipStream_t)stream;
+    if ((err = begin_call(st))) return err;
+    const size_t wsb = nn_workspace_bytes(points->size);
+    void *ws = pool_get(SLOT_NN, wsb, &err);
+    if (err) return err;
+    if ((err = launch_interp_nearest(points->x, points->y, points->z, points->size, bundle12(*values), 12,
+                                     grid->x, grid->y, grid->z, grid->size, bundle12(*out), ws, wsb, st)))
+        return err;
+    return end_call(st);
+}
+
 // ---- adjoint of the far-field power -------------------------------------------------
 
 int pypo_hip_displace_dev(const reflcontainer *base, const double *d, reflcontainer *out,

@@ -165,7 +165,7 @@ constexpr double kTwoPi = 6.283185307179586476925286766559;
 void *pool_get(int slot, size_t bytes, int *err);
 enum PoolSlot {
     SLOT_SRC_GRID = 0, SLOT_TGT_GRID, SLOT_CURRENTS, SLOT_RESULT, SLOT_PACKED,
-    SLOT_PARTIAL, SLOT_ERRS_S, SLOT_ERRS_T, SLOT_RAYS, SLOT_RT_ERRS, SLOT_ADJ, SLOT_FLAG,
+    SLOT_PARTIAL, SLOT_ERRS_S, SLOT_ERRS_T, SLOT_RAYS, SLOT_RT_ERRS, SLOT_ADJ, SLOT_FLAG, SLOT_NN,
     SLOT_COUNT
 };
 
@@ -249,6 +249,14 @@ int launch_rt_frame(const RTDict &r, const cframe &fr, hipStream_t st);
 int launch_grt_frame(const GRTDict &g, uint64_t seed, const cframe &fr, int *d_fail, int max_trials,
                      hipStream_t st);
 constexpr int kGrtMaxTrials = 1 << 20;
+
+// Hybrid ray-trace / PO (hybrid.hip).
+int launch_hybrid_phase(const cframe &in, const cframe &out, const Bundle12 &fin, const Bundle12 &fout,
+                        int n, double k, const double *start, hipStream_t st);
+size_t nn_workspace_bytes(int n_points);
+int launch_interp_nearest(const double *px, const double *py, const double *pz, int n, const Bundle12 &vin,
+                          int nvals, const double *qx, const double *qy, const double *qz, int m,
+                          const Bundle12 &vout, void *ws, size_t ws_bytes, hipStream_t st);
 
 // Kernel timing of the last timed launch (HIP events on the launch stream).
 void record_kernel_ms(double ms);

@@ -149,3 +149,72 @@ def tensor_to_complex6(t):
     a = t.detach().cpu().numpy()
     return np.stack([a[0] + 1j * a[3], a[1] + 1j * a[4], a[2] + 1j * a[5],
                      a[6] + 1j * a[9], a[7] + 1j * a[10], a[8] + 1j * a[11]])
+
+
+def _declare_hybrid(lib):
+    if getattr(lib, "_hybrid_declared", False):
+        return
+    P = ctypes.POINTER
+    lib.pypo_hip_hybrid_dev.argtypes = [PS.reflparams, P(PS.cframe), P(PS.cframe), P(PS.c2Bundle),
+                                        P(PS.c2Bundle), P(ctypes.c_double), ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_void_p]
+    lib.pypo_hip_hybrid_dev.restype = ctypes.c_int
+    lib.pypo_hip_interp_nearest_dev.argtypes = [P(PS.cframe), P(PS.c2Bundle), P(PS.reflcontainer),
+                                                P(PS.c2Bundle), ctypes.c_void_p]
+    lib.pypo_hip_interp_nearest_dev.restype = ctypes.c_int
+    lib._hybrid_declared = True
+
+
+def hybrid(target, frame, field, k, start=None, tol=1e-3, t0=1.0):
+    """Hybrid ray-trace / PO step: trace the (6, n) frame onto `target` and
+    carry the (12, n) field samples along the rays (exp(j k L), with the
+    start-point amplitude factor when `start` is given).  Returns the traced
+    frame and the carried field, both on the device."""
+    lib = B.loadHIPlib()
+    _declare_hybrid(lib)
+    _check_dev(frame, 6, "frame")
+    _check_dev(field, 12, "field")
+    if field.shape[1] != frame.shape[1]:
+        raise ValueError("frame and field must have the same number of points")
+    fr_out = torch.empty_like(frame)
+    f_out = torch.empty_like(field)
+    fi, fo = frame_struct(frame), frame_struct(fr_out)
+    bi, bo = bundle(field), bundle(f_out)
+    st = None
+    if start is not None:
+        st = (ctypes.c_double * 3)(*[float(v) for v in np.asarray(start, dtype=np.float64).ravel()[:3]])
+    rc = lib.pypo_hip_hybrid_dev(MU.fill_reflparams(target), ctypes.byref(fi), ctypes.byref(fo),
+                                 ctypes.byref(bi), ctypes.byref(bo), st, float(k), float(tol), float(t0),
+                                 _stream())
+    if rc:
+        B.check(lib, "pypo_hip_hybrid_dev")
+    return fr_out, f_out
+
+
+def interp_nearest(frame, values, grid):
+    """(12, m) values at the (7, m) grid's points: each takes the values of the
+    nearest point of the (6, n) frame (ties: smallest point index)."""
+    lib = B.loadHIPlib()
+    _declare_hybrid(lib)
+    _check_dev(frame, 6, "frame")
+    _check_dev(values, 12, "values")
+    if values.shape[1] != frame.shape[1]:
+        raise ValueError("frame and values must have the same number of points")
+    out = torch.empty((12, grid.shape[1]), dtype=torch.float64, device=grid.device)
+    fs, bv, g, bo = frame_struct(frame), bundle(values), container(grid), bundle(out)
+    rc = lib.pypo_hip_interp_nearest_dev(ctypes.byref(fs), ctypes.byref(bv), ctypes.byref(g),
+                                         ctypes.byref(bo), _stream())
+    if rc:
+        B.check(lib, "pypo_hip_interp_nearest_dev")
+    return out
+
+
+def frame_tensor(fr, device="cuda"):
+    """frame object -> (6, n) tensor."""
+    return to_device(np.stack([np.ravel(getattr(fr, a)) for a in ("x", "y", "z", "dx", "dy", "dz")]), device)
+
+
+def tensor_to_frame(t):
+    from . import types as T
+    a = t.detach().cpu().numpy()
+    return T.frame(a.shape[1], *[a[i].copy() for i in range(6)])

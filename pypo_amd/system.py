@@ -9,6 +9,13 @@ Checks.check_runRTDict (966-1058), and the EHP Poynting frame
 result containers and error classes.  Every propagation runs on the GPU
 through libpypohip.so; there is no CPU fallback.  Extra runPODict key:
 "precision" = "mixed" (default) or "fp64".
+
+Also mirrored: beam and frame creation (createTubeFrame 1243-1266,
+createGRTFrame 1268-1302, createGaussian 1373-1404, createGaussianBeam
+1406-1445, createScalarGaussian 1447-1472, createPointSource /
+createUniformSource and their scalar twins 2146-2311), calcRayLen
+(1304-1372), runHybridPropagation (1570-1622) and interpFrame (1624-1692,
+nearest-neighbour method).
 """
 
 import copy
@@ -31,6 +38,70 @@ class RunPOError(Exception):
 
 class RunRTError(Exception):
     """A runRTDict is invalid (Checks.py:248-256)."""
+
+
+class HybridPropError(Exception):
+    """A hybridDict is invalid (Checks.py:1604-1654)."""
+
+
+FIELD_COMPONENTS = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
+
+
+def _comp_index(comp):
+    """None / "NONE" -> None; "Ex".."Hz", 0..5 or an enum with .value -> index
+    (Enums.FieldComponents)."""
+    if comp is None:
+        return None
+    if hasattr(comp, "value"):
+        comp = comp.value
+        if comp is None:
+            return None
+    if isinstance(comp, str):
+        if comp.upper() == "NONE":
+            return None
+        return FIELD_COMPONENTS[comp]
+    c = int(comp)
+    if not 0 <= c <= 5:
+        raise ValueError(f"field component {comp} out of range")
+    return c
+
+
+def calc_beam(d):
+    """Fill w0 / z of a vecGPODict from any two of w0, z, w, R (Checks.calc_beam,
+    Checks.py:1299-1350)."""
+    lam = d["lam"]
+    if "w0" in d:
+        w0 = d["w0"]
+        if "z" in d:
+            return
+        if "w" in d:
+            d["z"] = np.pi * w0 / lam * np.sqrt(d["w"] ** 2 - w0 ** 2)
+            return
+        if "R" in d:
+            R = d["R"]
+            z = R / 2 * (1 + np.sqrt(1 - (2 * np.pi * w0 ** 2 / (lam * R)) ** 2))
+            if np.isnan(z):
+                z = R / 2 * (1 - np.sqrt(1 - (2 * np.pi * w0 ** 2 / (lam * R)) ** 2))
+            d["z"] = z
+            return
+    elif "z" in d:
+        z = d["z"]
+        if "w" in d:
+            w = d["w"]
+            w0 = np.sqrt(w ** 2 / 2 * (1 - np.sqrt(1 - ((2 * lam * z) / (np.pi * w ** 2)) ** 2)))
+            if np.isnan(w0):
+                w0 = np.sqrt(w ** 2 / 2 * (1 + np.sqrt(1 - ((2 * lam * z) / (np.pi * w ** 2)) ** 2)))
+            d["w0"] = w0
+            return
+        if "R" in d:
+            d["w0"] = np.sqrt(lam / np.pi * np.sqrt(z * (d["R"] - z)))
+            return
+    elif "w" in d and "R" in d:
+        w, R = d["w"], d["R"]
+        d["w0"] = w / np.sqrt(1 + ((np.pi * w ** 2) / (lam * R)) ** 2)
+        d["z"] = R / (1 + ((lam * R) / (np.pi * w ** 2)) ** 2)
+        return
+    raise ValueError("Need at least two of w0, w, z, R to define the Gaussian beam.")
 
 
 def _get_index(name, names, override):
@@ -92,6 +163,152 @@ class System:
 
     def addFrame(self, name, fr):
         self.frames[name] = fr
+
+    # ---- beams and frames (beam.py, device kernels) -------------------------------
+    def _name(self, d, default, pool):
+        d.setdefault("name", default)
+        num = _get_index(d["name"], pool.keys(), self.override)
+        if num > 0:
+            d["name"] = f"{d['name']}_{num}"
+        return d["name"]
+
+    def createTubeFrame(self, argDict):
+        """Tube of rays (System.createTubeFrame, System.py:1243-1266)."""
+        from . import beam as PB
+        d = copy.copy(argDict)
+        for key in ("nRays", "nRing"):
+            d[key] = abs(int(d[key]))
+        name = self._name(d, "TubeFrame", self.frames)
+        fr = PB.makeRTframe(d)
+        fr.setMeta(np.zeros(3), np.array([0.0, 0.0, 1.0]), np.eye(4))
+        self.frames[name] = fr
+        return fr
+
+    def createGRTFrame(self, argDict):
+        """Gaussian ray frame (System.createGRTFrame, System.py:1268-1302): the
+        angular spread follows from lam / (pi n w0)."""
+        from . import beam as PB
+        d = copy.copy(argDict)
+        d["nRays"] = abs(int(d["nRays"]))
+        d.setdefault("n", 1.0)
+        d.setdefault("seed", -1)
+        d["angx0"] = np.degrees(d["lam"] / (np.pi * d["n"] * d["x0"]))
+        d["angy0"] = np.degrees(d["lam"] / (np.pi * d["n"] * d["y0"]))
+        name = self._name(d, "GaussFrame", self.frames)
+        fr = PB.makeGRTframe(d)
+        fr.setMeta(np.zeros(3), np.array([0.0, 0.0, 1.0]), np.eye(4))
+        self.frames[name] = fr
+        return fr
+
+    def _surface(self, name):
+        if name not in self.system:
+            raise RunPOError(f"{name} is not a surface in the system")
+        return self.system[name]
+
+    def createGaussian(self, gaussDict, name_surface):
+        """Astigmatic Gaussian E field and its currents (System.py:1373-1404)."""
+        from . import beam as PB
+        d = copy.copy(gaussDict)
+        d.setdefault("n", 1.0)
+        d.setdefault("dxyz", 0.0)
+        d.setdefault("E0", 1.0)
+        d.setdefault("pol", np.array([1.0, 0.0, 0.0]))
+        name = self._name(d, "GaussianBeamPO", self.fields)
+        f, c = PB.makeGauss(d, self._surface(name_surface))
+        k = 2 * np.pi / d["lam"]
+        f.setMeta(name_surface, k)
+        c.setMeta(name_surface, k)
+        self.fields[name], self.currents[name] = f, c
+        return f, c
+
+    def createGaussianBeam(self, gaussDict, name_surface):
+        """Vector Gaussian by the complex source point (System.py:1406-1445)."""
+        from . import beam as PB
+        d = copy.copy(gaussDict)
+        calc_beam(d)
+        d.setdefault("n", 1.0)
+        d.setdefault("power", 4 * np.pi)
+        d.setdefault("mode", 0)
+        name = self._name(d, "GaussianBeamPO", self.fields)
+        f, c = PB.makeGaussBeam(d, self._surface(name_surface))
+        k = 2 * np.pi / d["lam"]
+        f.setMeta(name_surface, k)
+        c.setMeta(name_surface, k)
+        self.fields[name], self.currents[name] = f, c
+        return f, c
+
+    def createScalarGaussian(self, gaussDict, name_surface):
+        """Scalar Gaussian (System.py:1447-1472)."""
+        from . import beam as PB
+        d = copy.copy(gaussDict)
+        d.setdefault("n", 1.0)
+        d.setdefault("dxyz", 0.0)
+        d.setdefault("E0", 1.0)
+        name = self._name(d, "GaussianBeamPO", self.scalarfields)
+        s = PB.makeScalarGauss(d, self._surface(name_surface))
+        s.setMeta(name_surface, 2 * np.pi / d["lam"])
+        self.scalarfields[name] = s
+        return s
+
+    def _source_field(self, d, name_surface, point):
+        shape = tuple(int(v) for v in self._surface(name_surface)["gridsize"])
+        d.setdefault("E0", 1.0)
+        d.setdefault("phase", 0.0)
+        amp = d["E0"] * np.exp(1j * d["phase"])
+        if point:
+            a = np.zeros(shape, dtype=complex)
+            a[int((shape[0] - 1) / 2), int((shape[1] - 1) / 2)] = amp
+        else:
+            a = np.ones(shape, dtype=complex) * amp
+        return a
+
+    def _vector_source(self, sDict, name_surface, point, default):
+        from . import beam as PB
+        d = copy.copy(sDict)
+        d.setdefault("pol", np.array([1.0, 0.0, 0.0]))
+        a = self._source_field(d, name_surface, point)
+        z = a * 0
+        f = T.fields(a * d["pol"][0], a * d["pol"][1], a * d["pol"][2], z, z, z)
+        c = PB.calcCurrents(f, self.system[name_surface], "PMC")
+        k = 2 * np.pi / d["lam"]
+        name = self._name(d, default, self.fields)
+        f.setMeta(name_surface, k)
+        c.setMeta(name_surface, k)
+        self.fields[name], self.currents[name] = f, c
+        return f, c
+
+    def createPointSource(self, PSDict, name_surface):
+        """Point source at the surface grid's centre, PMC currents (System.py:2146-2194)."""
+        return self._vector_source(PSDict, name_surface, True, "PointSourcePO")
+
+    def createUniformSource(self, UDict, name_surface):
+        """Uniform field on the surface, PMC currents (System.py:2196-2237)."""
+        return self._vector_source(UDict, name_surface, False, "UniformSourcePO")
+
+    def createPointSourceScalar(self, PSDict, name_surface):
+        d = copy.copy(PSDict)
+        a = self._source_field(d, name_surface, True)
+        name = self._name(d, "PointSourcePO", self.scalarfields)
+        s = T.scalarfield(a)
+        s.setMeta(name_surface, 2 * np.pi / d["lam"])
+        self.scalarfields[name] = s
+        return s
+
+    def createUniformSourceScalar(self, UDict, name_surface):
+        d = copy.copy(UDict)
+        a = self._source_field(d, name_surface, False)
+        name = self._name(d, "UniformSourcePO", self.scalarfields)
+        s = T.scalarfield(a)
+        s.setMeta(name_surface, 2 * np.pi / d["lam"])
+        self.scalarfields[name] = s
+        return s
+
+    def calcCurrents(self, name_source, fields, mode="PEC"):
+        """Currents of a fields object on surface `name_source` (BindBeam.calcCurrents)."""
+        from . import beam as PB
+        c = PB.calcCurrents(fields, self._surface(name_source), mode)
+        c.setMeta(name_source, fields.k)
+        return c
 
     # ---- PO ----------------------------------------------------------------------
     def _check_runPODict(self, d):
@@ -246,3 +463,108 @@ class System:
             return np.full(3, np.nan)
         t = np.array([fr.dx[good].mean(), fr.dy[good].mean(), fr.dz[good].mean()])
         return t / np.linalg.norm(t)
+
+    # ---- hybrid ray-trace / PO ------------------------------------------------------
+    def calcRayLen(self, *frames, start=None):
+        """Ray lengths between consecutive frames, plus their sum (System.calcRayLen,
+        System.py:1304-1372); with `start`, the first entry is the distance from
+        that point to the first frame."""
+        for f in frames:
+            if f not in self.frames:
+                raise RunRTError(f"Frame {f} not in system.")
+        out = []
+        fr0 = self.frames[frames[0]]
+        total = np.zeros(len(fr0.x))
+        if isinstance(start, np.ndarray):
+            lens = np.sqrt((fr0.x - start[0]) ** 2 + (fr0.y - start[1]) ** 2 + (fr0.z - start[2]) ** 2)
+            out.append(lens)
+            total = total + lens
+        for name in frames[1:]:
+            fr = self.frames[name]
+            lens = np.sqrt((fr.x - fr0.x) ** 2 + (fr.y - fr0.y) ** 2 + (fr.z - fr0.z) ** 2)
+            out.append(lens)
+            total = total + lens
+            fr0 = fr
+        out.append(total)
+        return out
+
+    def interpFrame(self, name_fr_in, name_field, name_target, name_out, comp=None, method="nearest"):
+        """Interpolate a field carried by a frame onto surface `name_target`'s grid
+        (System.interpFrame, System.py:1624-1692) -- nearest neighbour, on the GPU."""
+        from . import device as D
+        if method != "nearest":
+            raise ValueError("interpFrame runs the nearest-neighbour method on the GPU only")
+        if name_fr_in not in self.frames:
+            raise RunRTError(f"Frame {name_fr_in} not in system.")
+        field = self.fields[name_field]
+        grid = D.generate_grid(self._surface(name_target))
+        out = D.interp_nearest(D.frame_tensor(self.frames[name_fr_in]), D.currents_tensor(field), grid)
+        shape = tuple(int(v) for v in self.system[name_target]["gridsize"])
+        comps = [c.reshape(shape) for c in D.tensor_to_complex6(out)]
+        ci = _comp_index(comp)
+        if ci is None:
+            res = T.fields(*comps)
+            ret = res
+        else:
+            null = np.zeros(shape, dtype=complex)
+            res = T.fields(*[comps[i] if i == ci else null for i in range(6)])
+            ret = comps[ci]
+        res.setMeta(name_target, field.k)
+        self.fields[name_out] = res
+        return ret
+
+    def _check_hybridDict(self, d):
+        try:
+            self._check_runRTDict(d)
+        except RunRTError as e:
+            raise HybridPropError(str(e))
+        err = []
+        if d.get("field_in") not in self.fields:
+            err.append("Please provide a valid field_in in hybridDict.")
+        if "field_out" not in d:
+            err.append("Please provide field_out in hybridDict.")
+        else:
+            num = _get_index(d["field_out"], self.fields.keys(), self.override)
+            if num > 0:
+                d["field_out"] = f"{d['field_out']}_{num}"
+        start = d.setdefault("start", None)
+        if start is not None:
+            start = np.asarray(start, dtype=np.float64)
+            if start.shape != (3,):
+                err.append("start must be a length-3 array.")
+            d["start"] = start
+        d.setdefault("interp", True)
+        if not isinstance(d["interp"], bool):
+            err.append("interp must be a bool.")
+        d.setdefault("comp", None)
+        if not err and d["field_in"] in self.fields:
+            surf = self.fields[d["field_in"]].surf
+            assoc = self.assoc.get(surf)
+            if assoc is not None and (assoc[0] != d["field_in"] or assoc[1] != d["fr_in"]):
+                err.append(f"{d['field_in']} and {d['fr_in']} are not associated on {surf}.")
+        if err:
+            for e in err:
+                log.error(e)
+            raise HybridPropError("\n".join(err))
+
+    def runHybridPropagation(self, hybridDict):
+        """Hybrid RT/PO step (System.runHybridPropagation, System.py:1570-1622): trace
+        fr_in onto t_name, carry field_in along the rays with exp(j k L) (and the
+        start-point amplitude), optionally interpolate onto t_name's grid."""
+        from . import device as D
+        d = copy.copy(hybridDict)
+        self._check_hybridDict(d)
+        field = self.fields[d["field_in"]]
+        fr_t, f_t = D.hybrid(self.system[d["t_name"]], D.frame_tensor(self.frames[d["fr_in"]]),
+                             D.currents_tensor(field), field.k, start=d["start"], tol=d["tol"], t0=d["t0"])
+        fr_out = D.tensor_to_frame(fr_t)
+        fr_out.setMeta(self.calcRTcenter(fr_out), self.calcRTtilt(fr_out), np.eye(4))
+        self.frames[d["fr_out"]] = fr_out
+        shape = np.asarray(field.Ex).shape
+        carried = T.fields(*[c.reshape(shape) for c in D.tensor_to_complex6(f_t)])
+        carried.setMeta(d["t_name"], field.k)
+        self.fields[d["field_out"]] = carried
+        if d["interp"]:
+            self.interpFrame(d["fr_out"], d["field_out"], d["t_name"], d["field_out"], comp=d["comp"])
+        self.assoc[d["t_name"]] = [d["field_out"], d["fr_out"]]
+        return self.fields[d["field_out"]]
