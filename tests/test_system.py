@@ -120,3 +120,64 @@ def test_runRayTracer_and_frame_metadata(hip, port):
     np.testing.assert_allclose(fr.pos, [x.mean(), y.mean(), fr.z.mean()], atol=1e-9)
     # reflected axis-parallel rays point at the focus on average: tilt ~ +z
     assert fr.ori[2] > 0.9
+
+
+def test_invalid_hybridDict_raises():
+    from pypo_amd.system import HybridPropError
+    s = make_system()
+    with pytest.raises(HybridPropError):
+        s.runHybridPropagation({"fr_in": "none", "t_name": "dish", "fr_out": "o", "field_in": "x",
+                                "field_out": "y"})
+
+
+def test_calc_beam_pairs():
+    """w0 and z of a vector Gaussian from the other pairs (Checks.calc_beam):
+    they reproduce the beam width w(z) = w0 sqrt(1 + (z/zR)^2) and the phase
+    radius R(z) = z (1 + (zR/z)^2), zR = pi w0^2 / lam."""
+    from pypo_amd.system import calc_beam
+    lam, w0, z = 1.0, 2.0, 30.0
+    zR = np.pi * w0 ** 2 / lam
+    w = w0 * np.sqrt(1 + (z / zR) ** 2)
+    R = z * (1 + (zR / z) ** 2)
+    for given in ({"w0": w0, "w": w}, {"z": z, "R": R}, {"w": w, "R": R}):
+        d = dict(given, lam=lam)
+        calc_beam(d)
+        assert d["w0"] == pytest.approx(w0, rel=1e-9)
+        assert d["z"] == pytest.approx(z, rel=1e-9)
+    with pytest.raises(ValueError):
+        calc_beam({"lam": 1.0, "w0": 1.0})
+
+
+@pytest.mark.gpu
+def test_system_beams_and_frames(hip, port):
+    """createTubeFrame / createGaussian / createPointSource / createUniformSource
+    / createScalarGaussian through System, against the CPU checker."""
+    import cases as C
+    s = make_system()
+    rt = C.beam_cases()["rt_tube_3ring"]["params"]
+    fr = s.createTubeFrame(dict(rt, name="tube"))
+    ref = OB.make_rt_frame(port, rt)
+    np.testing.assert_allclose(np.stack([fr.x, fr.y, fr.dz]), np.stack([ref.x, ref.y, ref.dz]), atol=1e-13)
+    g = {"name": "gauss", "lam": 0.8, "w0x": 1.0, "w0y": 1.2, "n": 1.0, "E0": 1.0, "dxyz": 0.0,
+         "pol": np.array([0.0, 1.0, 0.0])}
+    f, c = s.createGaussian(g, "feed")
+    fref, cref = OB.make_gauss(port, g, s.system["feed"])
+    assert group_err(flat(f), flat(fref)) <= 1e-12 and group_err(flat(c), flat(cref)) <= 1e-12
+    assert s.currents["gauss"].surf == "feed"
+    ps, pc = s.createPointSource({"name": "ps", "lam": 0.8, "E0": 2.0, "phase": 0.3,
+                                  "pol": np.array([1.0, 0.0, 0.0])}, "feed")
+    assert np.count_nonzero(flat(ps)) == 1
+    assert group_err(flat(pc), flat(OB.calc_currents(port, ps, s.system["feed"], 1))) <= 1e-14
+    us, uc = s.createUniformSource({"name": "us", "lam": 0.8, "E0": 1.0, "phase": 0.0,
+                                    "pol": np.array([0.0, 1.0, 0.0])}, "feed")
+    assert group_err(flat(uc), flat(OB.calc_currents(port, us, s.system["feed"], 1))) <= 1e-14
+    sg = {"name": "sg", "lam": 0.8, "w0x": 1.0, "w0y": 1.0, "n": 1.0, "E0": 1.0, "dxyz": 0.0}
+    sf = s.createScalarGaussian(sg, "feed")
+    sref = OB.make_scalar_gauss(port, sg, s.system["feed"])
+    assert np.max(np.abs(np.ravel(sf.S) - np.ravel(sref.S))) <= 1e-12 * np.max(np.abs(sref.S))
+    # the Gaussian's currents drive the PO chain like any other currents
+    out = s.runPO({"t_name": "dish", "name_JM": "jm", "s_current": "gauss", "mode": "JM", "precision": "fp64"})
+    ref_jm, _, _ = OB.propagate(port, "JM", s.system["feed"], s.system["dish"], cref, 2 * np.pi / 0.8)
+    assert group_err(flat(out), flat(ref_jm)) <= 1e-10
+    lens = s.calcRayLen("tube", "tube")
+    assert np.all(lens[0] == 0) and len(lens) == 2
