@@ -41,23 +41,46 @@ struct FrameSet {
     FramePtrs f[MAX_SURF + 1];
 };
 
+// 1/d from v_rcp_f64 and two Newton-Raphson steps (about 1 ulp); inf/NaN
+// inputs give NaN, which the miss rule treats like the reference's inf/NaN.
+__device__ __forceinline__ double rcp_nr(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = __builtin_fma(__builtin_fma(-d, r, 1.0), r, r);
+    r = __builtin_fma(__builtin_fma(-d, r, 1.0), r, r);
+    return r;
+}
+
+// 1/sqrt(s) from v_rsq_f64 and two Newton-Raphson steps (about 1 ulp);
+// negative s gives NaN like sqrt.
+__device__ __forceinline__ double rsq_nr(double s) {
+    double y = __builtin_amdgcn_rsq(s);
+    const double h = 0.5 * s;
+    y = y * __builtin_fma(-h * y, y, 1.5);
+    y = y * __builtin_fma(-h * y, y, 1.5);
+    return y;
+}
+
 // One Newton update t <- t - f(t)/f'(t) for the quadric (RTRefls.h gp 96,
 // gh 120, ge 145, gpl 170).  common1/common2 (RTRefls.h:52-94) are
-// evaluated once per step with the reciprocal squared semi-axes; this moves
-// intermediate values by at most an ulp against the reference's x/(a*a).
+// evaluated once per step with the reciprocal squared semi-axes.  f(t) is
+// evaluated to about an ulp (its zero is the converged t); the step
+// f/f' uses a refined reciprocal instead of an IEEE division, and for the
+// hyperboloid and ellipsoid one refined 1/sqrt gives both sqrt(.) = s/sqrt(s)
+// and the 1/(2 sqrt(.)) of the derivative.  These change t only at rounding
+// level against the reference's divisions and sqrt.
 __device__ __forceinline__ double newton(const Surf &s, double t, double x, double y, double z,
                                          double dx, double dy, double dz) {
     const double X = x + t * dx, Y = y + t * dy;
-    if (s.type == 3) return t - (z + t * dz) / dz;
+    if (s.type == 3) return t - (z + t * dz) * rcp_nr(dz);
     const double c1 = X * X * s.ia2 + Y * Y * s.ib2;
     const double c2 = 2.0 * (X * dx * s.ia2 + Y * dy * s.ib2);
-    if (s.type == 0) return t - (z + t * dz - c1) / (dz - c2);
+    if (s.type == 0) return t - (z + t * dz - c1) * rcp_nr(dz - c2);
     if (s.type == 1) {
-        const double r = sqrt(c1 + 1);
-        return t - (z + t * dz - s.c * r) / (dz - s.c / (2 * r) * c2);
+        const double q = c1 + 1, iy = rsq_nr(q), r = q * iy;
+        return t - (z + t * dz - s.c * r) * rcp_nr(dz - 0.5 * s.c * iy * c2);
     }
-    const double r = sqrt(1 - c1);
-    return t - (z + t * dz - s.c * r) / (dz + s.c / (2 * r) * c2);
+    const double q = 1 - c1, iy = rsq_nr(q), r = q * iy;
+    return t - (z + t * dz - s.c * r) * rcp_nr(dz + 0.5 * s.c * iy * c2);
 }
 
 __global__ __launch_bounds__(256) void k_trace(SurfSet S, FrameSet F, int n, double tol, double t0,
@@ -114,8 +137,8 @@ __global__ __launch_bounds__(256) void k_trace(SurfSet S, FrameSet F, int n, dou
                     nx = 2 * px * s.ia2 * fl; ny = 2 * py * s.ib2 * fl;
                     nz = 2 * pz * s.ic2 * fl;
                 }
-                double nrm = sqrt(nx * nx + ny * ny + nz * nz);
-                nx /= nrm; ny /= nrm; nz /= nrm;
+                const double inrm = rsq_nr(nx * nx + ny * ny + nz * nz);
+                nx *= inrm; ny *= inrm; nz *= inrm;
             }
             double f = 2. * (dx * nx + dy * ny + dz * nz);
             odx = dx - nx * f; ody = dy - ny * f; odz = dz - nz * f;
