@@ -113,6 +113,10 @@ def _get_index(name, names, override):
     return max(nums) + 1 if nums else 0
 
 
+# Limit keys of each grid mode (0 xy, 1 uv, 2 AoE), scaled by autoConverge.
+_LIMS = {0: ("lims_x", "lims_y"), 1: ("lims_u", "lims_v"), 2: ("lims_Az", "lims_El")}
+
+
 class System:
     def __init__(self, override=True):
         self.system = {}
@@ -568,3 +572,144 @@ class System:
             self.interpFrame(d["fr_out"], d["field_out"], d["t_name"], d["field_out"], comp=d["comp"])
         self.assoc[d["t_name"]] = [d["field_out"], d["fr_out"]]
         return self.fields[d["field_out"]]
+
+    # ---- grid-convergence loops (System.py:2779-3080) ------------------------------
+    def copyElement(self, name, name_copy):
+        self.system[name_copy] = copy.deepcopy(self.system[name])
+        self.system[name_copy]["name"] = name_copy
+
+    def removeElement(self, name):
+        self.system.pop(name, None)
+
+    def removeScalarField(self, name):
+        self.scalarfields.pop(name, None)
+
+    def _strongest_component(self, source_field):
+        """The source's component with the largest magnitude, as a scalarfield
+        registered under "_" + source_field (System.py:2799-2803, 2924-2942)."""
+        if source_field in self.fields:
+            obj = self.fields[source_field]
+        elif source_field in self.currents:
+            obj = self.currents[source_field]
+        elif source_field in self.scalarfields:
+            sf = self.scalarfields[source_field]
+            self.scalarfields["_" + source_field] = sf
+            return
+        else:
+            raise RunPOError(f"{source_field} is not a field, current or scalar field in the system")
+        comps = list(obj.components())
+        best = int(np.argmax([np.max(np.abs(c)) for c in comps]))
+        sf = T.scalarfield(comps[best])
+        sf.setMeta(obj.surf, obj.k)
+        self.scalarfields["_" + source_field] = sf
+
+    def autoConverge(self, source_field, name_target, tol=1e-2, add=10, patch_size=1 / 9, max_iter=1000):
+        """Grid size of `name_target` at which the scalar power collected on a
+        central patch stops changing by more than tol (System.autoConverge,
+        System.py:2779-2880).  The patch is the target's limits times patch_size;
+        its grid grows by `add` per step; the returned size is scaled back to the
+        full surface and set on it.  Every propagation is a GPU runPO."""
+        self._strongest_component(source_field)
+        tgt = self.system[name_target]
+        xu, yv = _LIMS[int(tgt["gmode"])]
+        orig = (tgt[xu], tgt[yv])
+        tgt[xu] = patch_size * np.asarray(orig[0], dtype=float)
+        tgt[yv] = patch_size * np.asarray(orig[1], dtype=float)
+        gridsize = np.array([1, 1])
+        P0, diff, norm, n = 1e99, 1e99, 1.0, 0
+        d = {"t_name": name_target, "s_scalarfield": "_" + source_field, "epsilon": 10, "exp": "fwd",
+             "mode": "scalar", "name_field": "_S_conv"}
+        override, self.override = self.override, True
+        try:
+            while np.abs(diff) > tol:
+                gridsize = gridsize + add
+                tgt["gridsize"] = gridsize
+                S = self.runPO(dict(d)).S
+                if n == 0:
+                    norm = np.max(np.abs(S))
+                grid = self.generateGrids(name_target)
+                P = np.sum(np.abs(S / norm) ** 2 * grid.area)
+                diff = np.abs(P0 - P)
+                P0 = P
+                self.removeScalarField("_S_conv")
+                log.info(f"Difference : {diff} at gridsize {gridsize}")
+                n += 1
+                if n >= max_iter:
+                    log.error("Could not find converged solution.")
+                    break
+        finally:
+            self.override = override
+            tgt[xu], tgt[yv] = orig
+        gridsize = (gridsize / patch_size).astype(int)
+        tgt["gridsize"] = gridsize
+        return gridsize
+
+    def convergeOnTarget(self, source_field, name_scatterer, name_target, tol=1e-2, target_gridsize=None,
+                         mult=2, div=3, max_iter=16):
+        """Grid size of `name_scatterer` at which the scalar field it scatters onto
+        a few test points of `name_target` converges (System.convergeOnTarget,
+        System.py:2882-3080): per axis, multiply the size by `mult` until two
+        steps differ by less than tol (max relative to the peak), then bisect
+        `div` times between the last unconverged and first converged sizes."""
+        self._strongest_component(source_field)
+        self.copyElement(name_target, "_conv_target")
+        gmode = int(self.system["_conv_target"]["gmode"])
+        if target_gridsize is not None:
+            self.system["_conv_target"]["gridsize"] = np.asarray(target_gridsize)
+        else:
+            self.system["_conv_target"]["gridsize"] = np.array((3, 6)) if gmode == 1 else np.array((5, 5))
+        self.copyElement(name_scatterer, "_conv_scatterer")
+        scat = self.system["_conv_scatterer"]
+        scat["gridsize"] = np.array((10, 30)) if gmode == 1 else np.array((10, 10))
+        d1 = {"t_name": "_conv_scatterer", "s_scalarfield": "_" + source_field, "mode": "scalar",
+              "name_field": "_S_conv_scat"}
+        d2 = {"t_name": "_conv_target", "s_scalarfield": "_S_conv_scat", "mode": "scalar",
+              "name_field": "_S_conv_target"}
+
+        def target_field():
+            self.runPO(dict(d1))
+            return self.runPO(dict(d2)).S.copy()
+
+        override, self.override = self.override, True
+        conv_grid = np.array([10, 10])
+        try:
+            init_field = target_field()
+            for i in range(2):
+                field, n, diff = init_field.copy(), 0, 1e99
+                scat["gridsize"] = np.array([10, 10])
+                gridsize = 10
+                while np.abs(diff) > tol:
+                    last_field = field.copy()
+                    gridsize = int(scat["gridsize"][i] * mult)
+                    scat["gridsize"][i] = gridsize
+                    field = target_field()
+                    diff = np.max(np.abs(np.abs(field - last_field))) / np.max(np.abs(field))
+                    log.info(f"Axis {i} Step {n + 1}: Difference {diff:.3e} at gridsize[{gridsize}]")
+                    n += 1
+                    if n >= max_iter:
+                        break
+                if n >= max_iter:
+                    log.error(f"Could not find converged solution for axis {i}; returning {gridsize}")
+                    conv_grid[i] = gridsize
+                    break
+                conv, nonconv = gridsize, gridsize / mult
+                conv_field = last_field.copy()
+                for j in range(div):
+                    nxt = int((conv + nonconv) / 2)
+                    scat["gridsize"][i] = nxt
+                    field = target_field()
+                    diff = np.max(np.abs(np.abs(field - conv_field))) / np.max(np.abs(field))
+                    log.info(f"Axis {i} Step {n + j + 1}: Difference {diff:.3e} at gridsize[{nxt}]")
+                    if diff < tol:
+                        conv, conv_field = nxt, field.copy()
+                    else:
+                        nonconv = nxt
+                conv_grid[i] = conv
+        finally:
+            self.override = override
+            for name in ("_S_conv_scat", "_S_conv_target"):
+                self.removeScalarField(name)
+            self.removeElement("_conv_target")
+            self.removeElement("_conv_scatterer")
+        self.system[name_scatterer]["gridsize"] = conv_grid
+        return conv_grid
