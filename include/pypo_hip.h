@@ -333,6 +333,48 @@ int pypo_hip_interp_nearest_dev(const cframe *points, const c2Bundle *values,
  * rounding.  Returns 0, or ERR_ARG for values out of range. */
 int pypo_hip_set_sum_shape(int tpl, int want_wgs);
 
+/* ---- Figures of merit on device data ---------------------------------------
+ * Reductions behind src/PyPO/Efficiencies.py, for fields and frames that are
+ * already in device memory (none has a C counterpart in the reference, whose
+ * efficiencies are numpy code).  Each writes a few doubles to HOST memory and
+ * returns after the stream has finished them.  FP64, deterministic order.
+ *
+ * Aperture of Efficiencies._generateMask (Efficiencies.py:11-41), the aperDict
+ * keys "shape", "center", "outer", "outer_x", "outer_y".  Ellipse: a point
+ * (x, y) of the surface's rest-frame grid is inside when
+ * (x-cx)^2 + (y-cy)^2 < (outer0 cos t)^2 + (outer1 sin t)^2, t = atan2(y, x) + pi;
+ * rectangle: outer_x0 < x-cx < outer_x1 and outer_y0 < y-cy < outer_y1 (open);
+ * NONE selects every point.  The reference's "inner" keys select nothing. */
+#define PYPO_APER_ELLIPSE 0
+#define PYPO_APER_RECT 1
+#define PYPO_APER_NONE 2
+typedef struct {
+    int shape;
+    double center[2];
+    double outer[2];
+    double outer_x[2];
+    double outer_y[2];
+} aperparams;
+
+/* sums[5] = { sum |f|^2 dA, sum_in |f|^2 dA, sum_in Re f dA, sum_in Im f dA, sum_in dA }
+ * over the grid (rest frame: pypo_hip_generate_grid_dev with an identity
+ * transf) and one complex field component f = f_re + j f_im.  Serves
+ * Efficiencies.calcSpillover (90-115: s1 / s0) and calcTaper (117-145:
+ * (s2^2 + s3^2) / s1 / s4). */
+int pypo_hip_aperture_sums_dev(const reflcontainer *grid, const double *f_re, const double *f_im,
+                               aperparams aper, double *sums, void *stream);
+
+/* sums[2] = { sum |a|^2, sum |b|^2 } over n points: Efficiencies.calcXpol
+ * (147-159) gives 1 - sums[1] / (sums[0] + sums[1]). */
+int pypo_hip_power_sums_dev(const double *a_re, const double *a_im, const double *b_re, const double *b_im,
+                            int n, double *sums, void *stream);
+
+/* stats[8] over the rays of a frame with |d|^2 > 0.8: { count, sum x, sum y,
+ * sum z, sum dx, sum dy, sum dz, sum |p - c|^2 } with c the mean position.
+ * Efficiencies.calcRTcenter (43-57) = sums/count, calcRTtilt (59-73) = the
+ * normalised mean direction, calcRMS (75-88) = sqrt(stats[7] / count). */
+int pypo_hip_frame_stats_dev(const cframe *fr, double *stats, void *stream);
+
 /* Mean kernel time (ms) of the last pypo_hip_propagate_dev / trace call's
  * dominant kernel, measured with HIP events on the launch stream, when
  * timing is enabled by pypo_hip_set_timing(1).  Returns -1 if unavailable. */

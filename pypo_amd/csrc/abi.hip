@@ -924,4 +924,74 @@ int pypo_hip_ff_power_grad_dev(const reflcontainer *dish, int gmode, int ncx, in
     return end_call(st);
 }
 
+// ---- figures of merit (Efficiencies.py) -------------------------------------------------
+
+// Device workspace of the reductions and the address of its result slots.
+static int metrics_ws(double **ws, double **res) {
+    int err = 0;
+    *ws = (double *)pool_get(SLOT_METRICS, metrics_workspace_bytes(), &err);
+    if (err) return err;
+    *res = *ws + metrics_result_offset();
+    return 0;
+}
+
+// Copies nres folded results to the host and waits for them.
+static int metrics_finish(hipStream_t st, const double *res, double *host, int nres) {
+    PYPO_HIP_TRY(hipMemcpyAsync(host, res, nres * sizeof(double), hipMemcpyDeviceToHost, st));
+    int err = end_call(st);
+    if (err) return err;
+    PYPO_HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int pypo_hip_aperture_sums_dev(const reflcontainer *grid, const double *f_re, const double *f_im,
+                               aperparams aper, double *sums, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    if ((err = check_container(grid, "grid"))) return err;
+    PYPO_REQUIRE(f_re && f_im && sums, ERR_ARG, "aperture sums: NULL field or output");
+    PYPO_REQUIRE(grid->size > 0, ERR_ARG, "aperture sums: empty grid");
+    PYPO_REQUIRE(aper.shape == PYPO_APER_ELLIPSE || aper.shape == PYPO_APER_RECT || aper.shape == PYPO_APER_NONE,
+                 ERR_ARG, "aperture sums: unknown aperture shape");
+    hipStream_t st = (hipStream_t)stream;
+    if ((err = begin_call(st))) return err;
+    double *ws, *res;
+    if ((err = metrics_ws(&ws, &res))) return err;
+    if ((err = launch_aperture_sums(grid->x, grid->y, grid->area, f_re, f_im, grid->size, aper, ws, res, st)))
+        return err;
+    return metrics_finish(st, res, sums, 5);
+}
+
+int pypo_hip_power_sums_dev(const double *a_re, const double *a_im, const double *b_re, const double *b_im,
+                            int n, double *sums, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    PYPO_REQUIRE(a_re && a_im && b_re && b_im && sums, ERR_ARG, "power sums: NULL argument");
+    PYPO_REQUIRE(n > 0, ERR_ARG, "power sums: n must be positive");
+    hipStream_t st = (hipStream_t)stream;
+    if ((err = begin_call(st))) return err;
+    double *ws, *res;
+    if ((err = metrics_ws(&ws, &res))) return err;
+    if ((err = launch_power_sums(a_re, a_im, b_re, b_im, n, ws, res, st))) return err;
+    return metrics_finish(st, res, sums, 2);
+}
+
+int pypo_hip_frame_stats_dev(const cframe *fr, double *stats, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    if ((err = check_frame(fr, "frame"))) return err;
+    PYPO_REQUIRE(stats, ERR_ARG, "frame stats: NULL output");
+    PYPO_REQUIRE(fr->size > 0, ERR_ARG, "frame stats: empty frame");
+    hipStream_t st = (hipStream_t)stream;
+    if ((err = begin_call(st))) return err;
+    double *ws, *res;
+    if ((err = metrics_ws(&ws, &res))) return err;
+    const FrameView f{fr->x, fr->y, fr->z, fr->dx, fr->dy, fr->dz};
+    if ((err = launch_frame_stats(f, fr->size, ws, res, st))) return err;
+    return metrics_finish(st, res, stats, 8);
+}
+
 }  // extern "C"

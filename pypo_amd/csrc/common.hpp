@@ -165,7 +165,7 @@ constexpr double kTwoPi = 6.283185307179586476925286766559;
 void *pool_get(int slot, size_t bytes, int *err);
 enum PoolSlot {
     SLOT_SRC_GRID = 0, SLOT_TGT_GRID, SLOT_CURRENTS, SLOT_RESULT, SLOT_PACKED,
-    SLOT_PARTIAL, SLOT_ERRS_S, SLOT_ERRS_T, SLOT_RAYS, SLOT_RT_ERRS, SLOT_ADJ, SLOT_FLAG, SLOT_NN,
+    SLOT_PARTIAL, SLOT_ERRS_S, SLOT_ERRS_T, SLOT_RAYS, SLOT_RT_ERRS, SLOT_ADJ, SLOT_FLAG, SLOT_NN, SLOT_METRICS,
     SLOT_COUNT
 };
 
@@ -257,6 +257,19 @@ size_t nn_workspace_bytes(int n_points);
 int launch_interp_nearest(const double *px, const double *py, const double *pz, int n, const Bundle12 &vin,
                           int nvals, const double *qx, const double *qy, const double *qz, int m,
                           const Bundle12 &vout, void *ws, size_t ws_bytes, hipStream_t st);
+
+// ---- figures of merit (metrics.hip) ----
+struct FrameView {
+    const double *x, *y, *z, *dx, *dy, *dz;
+};
+using AperMask = aperparams;
+size_t metrics_workspace_bytes();
+size_t metrics_result_offset();   // in doubles
+int launch_aperture_sums(const double *x, const double *y, const double *area, const double *fr,
+                         const double *fi, int n, const AperMask &m, double *ws, double *out5, hipStream_t st);
+int launch_power_sums(const double *ar, const double *ai, const double *br, const double *bi, int n, double *ws,
+                      double *out2, hipStream_t st);
+int launch_frame_stats(const FrameView &f, int n, double *ws, double *out8, hipStream_t st);
 
 // Kernel timing of the last timed launch (HIP events on the launch stream).
 void record_kernel_ms(double ms);

@@ -98,3 +98,10 @@ class ScalarGPODict(ctypes.Structure):
     """Scalar Gaussian beam parameters (Structs.h:206-213)."""
     _fields_ = [("lam", ctypes.c_double), ("w0x", ctypes.c_double), ("w0y", ctypes.c_double),
                 ("n", ctypes.c_double), ("E0", ctypes.c_double), ("dxyz", ctypes.c_double)]
+
+
+class aperparams(ctypes.Structure):
+    """Aperture of Efficiencies._generateMask (Efficiencies.py:11-41); shape
+    0 ellipse, 1 rectangle, 2 none (every point)."""
+    _fields_ = [("shape", ctypes.c_int), ("center", ctypes.c_double * 2), ("outer", ctypes.c_double * 2),
+                ("outer_x", ctypes.c_double * 2), ("outer_y", ctypes.c_double * 2)]

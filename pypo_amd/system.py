@@ -15,7 +15,11 @@ createGRTFrame 1268-1302, createGaussian 1373-1404, createGaussianBeam
 1406-1445, createScalarGaussian 1447-1472, createPointSource /
 createUniformSource and their scalar twins 2146-2311), calcRayLen
 (1304-1372), runHybridPropagation (1570-1622) and interpFrame (1624-1692,
-nearest-neighbour method).
+nearest-neighbour method); the frame and beam figures of merit
+(calcRTcenter / calcRTtilt / calcSpotRMS 1694-1746, calcSpillover / calcTaper
+/ calcXpol 1748-1826) as GPU reductions; findRTfocus (2632-2700) over GPU
+traces; and the grid-convergence loops autoConverge / convergeOnTarget
+(2779-3080) over GPU scalar propagations.
 """
 
 import copy
@@ -451,22 +455,119 @@ class System:
         self.frames[name_out] = fr
         return fr
 
-    @staticmethod
-    def calcRTcenter(fr):
-        """Centroid of the rays that hit (Efficiencies.calcRTcenter 43-57)."""
-        good = (fr.dx ** 2 + fr.dy ** 2 + fr.dz ** 2) > 0.8
-        if not np.any(good):
-            return np.full(3, np.nan)
-        return np.array([fr.x[good].mean(), fr.y[good].mean(), fr.z[good].mean()])
+    def _frame(self, name_frame):
+        if isinstance(name_frame, str):
+            if name_frame not in self.frames:
+                raise RunRTError(f"Frame {name_frame} not in system.")
+            return self.frames[name_frame]
+        return name_frame
 
+    def calcRTcenter(self, name_frame):
+        """Centroid of the rays that hit (System.calcRTcenter, System.py:1694-1710;
+        Efficiencies.calcRTcenter 43-57), reduced on the GPU.  Takes a frame's
+        name or a frame."""
+        from . import efficiencies as E
+        return E.calcRTcenter(self._frame(name_frame))
+
+    def calcRTtilt(self, name_frame):
+        """Normalised mean direction of the rays that hit (System.py:1712-1728)."""
+        from . import efficiencies as E
+        return E.calcRTtilt(self._frame(name_frame))
+
+    def calcSpotRMS(self, name_frame):
+        """RMS spot size of a frame (System.calcSpotRMS, System.py:1730-1746)."""
+        from . import efficiencies as E
+        return E.calcRMS(self._frame(name_frame))
+
+    # ---- efficiencies (System.py:1748-1826, Efficiencies.py) ----------------------------
+    def _field_comp(self, name_field, comp):
+        if name_field not in self.fields:
+            raise RunPOError(f"{name_field} is not a field in the system")
+        f = self.fields[name_field]
+        c = _comp_index(comp)
+        if c is None:
+            raise ValueError("a field component is required")
+        return f, f[c]
+
+    def calcSpillover(self, name_field, comp, aperDict):
+        """Spillover efficiency of a field component on an aperture of the field's surface."""
+        from . import efficiencies as E
+        f, fc = self._field_comp(name_field, comp)
+        return E.calcSpillover(fc, self.system[f.surf], aperDict)
+
+    def calcTaper(self, name_field, comp, aperDict=None):
+        """Taper efficiency of a field component (over the whole surface without aperDict)."""
+        from . import efficiencies as E
+        f, fc = self._field_comp(name_field, comp)
+        return E.calcTaper(fc, self.system[f.surf], aperDict or None)
+
+    def calcXpol(self, name_field, comp_co, comp_cr):
+        """Cross-polar efficiency of two components of a field."""
+        from . import efficiencies as E
+        _, co = self._field_comp(name_field, comp_co)
+        _, cr = self._field_comp(name_field, comp_cr)
+        return E.calcXpol(co, cr)
+
+    # ---- ray-trace focus (System.findRTfocus 2632-2700, _optimiseFocus 3217-3238) ------
     @staticmethod
-    def calcRTtilt(fr):
-        """Mean direction of the rays that hit (Efficiencies.calcRTtilt 59-73)."""
-        good = (fr.dx ** 2 + fr.dy ** 2 + fr.dz ** 2) > 0.8
-        if not np.any(good):
-            return np.full(3, np.nan)
-        t = np.array([fr.dx[good].mean(), fr.dy[good].mean(), fr.dz[good].mean()])
-        return t / np.linalg.norm(t)
+    def findRotation(v, u):
+        """4x4 rotation taking direction v onto u, by Rodrigues' formula about
+        v x u (System.findRotation, System.py:2714-2745)."""
+        v, u = np.asarray(v, dtype=float), np.asarray(u, dtype=float)
+        out = np.eye(4)
+        if np.array_equal(v, u):
+            return out
+        lv, lu = np.linalg.norm(v), np.linalg.norm(u)
+        if lv == 0 or lu == 0:
+            log.error("Encountered 0-length vector. Cannot proceed.")
+            return None
+        w = np.cross(v / lv, u / lu)
+        if not np.any(w) and np.dot(v, u) > 0:
+            return out   # parallel but not bit-equal: the reference's formula divides 0 by 0 here
+        K = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+        out[:3, :3] = np.eye(3) + K + K @ K * (1 - np.dot(v, u)) / np.linalg.norm(w) ** 2
+        return out
+
+    def findRTfocus(self, name_frame, f0=None, tol=1e-12):
+        """Point of least RMS spot size along the mean direction of a frame.
+
+        A 3x3-point plane focal_plane_<name> is placed through the frame's
+        centre, perpendicular to its mean direction; Nelder-Mead (scipy fmin,
+        ftol = tol) moves it along that direction, each trial tracing the
+        frame onto it on the GPU and reducing the spot RMS there, so only one
+        number per trial leaves the device.  The plane ends at the focus and
+        the frame traced onto it is stored as focus_<name>.  Returns the
+        focus."""
+        from scipy.optimize import fmin
+
+        from . import device as D
+        from . import efficiencies as E
+        fr = self._frame(name_frame)
+        f0 = 0.0 if f0 is None else f0
+        tilt, center = self.calcRTtilt(fr), self.calcRTcenter(fr)
+        match = np.array([0.0, 0.0, 1.0])
+        R = np.eye(4) if np.all(np.isclose(match, -tilt)) else self.findRotation(match, tilt)
+        t_name, fr_out = f"focal_plane_{name_frame}", f"focus_{name_frame}"
+        self.addPlane({"name": t_name, "gmode": "xy", "lims_x": np.array([-4.2, 4.2]),
+                       "lims_y": np.array([-4.2, 4.2]), "gridsize": np.array([3, 3])})
+        plane = self.system[t_name]
+        rays = D.frame_tensor(fr)
+
+        def place(p):
+            T4 = R.copy()
+            T4[:3, 3] = p
+            plane["transf"] = T4
+
+        def spot(f):
+            place(center + float(np.ravel(f)[0]) * tilt)
+            return E.calcRMS(D.trace([plane], rays, tol=tol)[0])
+
+        res = fmin(spot, f0, full_output=True, disp=False, ftol=tol)
+        out = res[0][0] * tilt + center
+        place(out)
+        self.frames[fr_out] = D.tensor_to_frame(D.trace([plane], rays, tol=tol)[0])
+        log.info(f"Focus of frame {name_frame}: {out}, RMS: {res[1]:.3e}")
+        return out
 
     # ---- hybrid ray-trace / PO ------------------------------------------------------
     def calcRayLen(self, *frames, start=None):
