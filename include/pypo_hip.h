@@ -333,6 +333,15 @@ int pypo_hip_interp_nearest_dev(const cframe *points, const c2Bundle *values,
  * rounding.  Returns 0, or ERR_ARG for values out of range. */
 int pypo_hip_set_sum_shape(int tpl, int want_wgs);
 
+/* Kernel of the mixed-precision far-field sum: PYPO_FF_MATRIX (default) runs
+ * the source products on the matrix cores (FP16 hi/lo operand split, FP32
+ * accumulation), PYPO_FF_VECTOR on the vector ALUs (FP32 FMAs).  Both fold
+ * per-tile sums into FP64 and meet the same accuracy bar; a tuning and
+ * cross-check hook.  Returns 0, or ERR_ARG. */
+#define PYPO_FF_MATRIX 0
+#define PYPO_FF_VECTOR 1
+int pypo_hip_set_ff_kernel(int kind);
+
 /* ---- Figures of merit on device data ---------------------------------------
  * Reductions behind src/PyPO/Efficiencies.py, for fields and frames that are
  * already in device memory (none has a C counterpart in the reference, whose

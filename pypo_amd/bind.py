@@ -96,6 +96,8 @@ def loadHIPlib():
         lib.pypo_hip_trace_dev.restype = i
         lib.pypo_hip_set_sum_shape.argtypes = [i, i]
         lib.pypo_hip_set_sum_shape.restype = i
+        lib.pypo_hip_set_ff_kernel.argtypes = [i]
+        lib.pypo_hip_set_ff_kernel.restype = i
         RC, CB = P(PS.reflcontainer), P(PS.c2Bundle)
         lib.pypo_hip_displace_dev.argtypes = [RC, P(d), RC, ctypes.c_void_p]
         lib.pypo_hip_displace_dev.restype = i
@@ -217,3 +219,17 @@ def generate_grid(refl, transform=True, spheric=True):
         _run(lib, lib.generateGrid, [MU.fill_reflparams(refl), ctypes.byref(c), bool(transform),
                                      bool(spheric)], "generateGrid")
     return MU.container_to_grids(c, (nx, ny))
+
+
+FF_KERNELS = {"matrix": 0, "vector": 1}
+
+
+def set_ff_kernel(kind):
+    """Kernel of the mixed-precision far-field sum: "matrix" (matrix cores,
+    the default) or "vector" (vector ALUs).  Returns nothing; raises on an
+    unknown name."""
+    lib = loadHIPlib()
+    if kind not in FF_KERNELS:
+        raise ValueError(f"far-field kernel must be one of {sorted(FF_KERNELS)}")
+    if lib.pypo_hip_set_ff_kernel(FF_KERNELS[kind]):
+        check(lib, "pypo_hip_set_ff_kernel")

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpypohip.so")
-SOURCES = ["abi.hip", "grid.hip", "po_pack.hip", "po_ff.hip", "po_near.hip", "po_scalar.hip",
+SOURCES = ["abi.hip", "grid.hip", "po_pack.hip", "po_ff.hip", "po_ff_mfma.hip", "po_near.hip", "po_scalar.hip",
            "po_launch.hip", "rt.hip", "sweep.hip", "adjoint.hip", "beam.hip", "hybrid.hip", "metrics.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -689,6 +689,11 @@ int pypo_hip_set_sum_shape(int tpl, int want_wgs) {
     return set_sum_shape(tpl, want_wgs);
 }
 
+int pypo_hip_set_ff_kernel(int kind) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    return set_ff_kernel(kind);
+}
+
 // ---- beam initialisation (InterfaceBeam.h:21-34) -----------------------------------
 
 void makeGauss(GPODict gdict, reflparams plane, c2Bundle *res_field, c2Bundle *res_current) {

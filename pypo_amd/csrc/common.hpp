@@ -86,9 +86,13 @@ static_assert(sizeof(SrcRec32) == 96, "SrcRec32 must be 96 bytes");
 // the cosine of its half-angle; -1 when no cone under 90 degrees exists) and
 // whether any magnetic current in the tile is non-zero.  Lets the far-field
 // sum skip the per-pair shadow test and the M terms where they cannot matter.
+// cmax: the largest |w A current| component of the tile (FP32 records), the
+// scale of the matrix-core far-field sum's FP16 operands.
 struct __attribute__((aligned(16))) TileMeta {
     float ax, ay, az, cos_half;
-    int has_m, n_valid, pad0, pad1;
+    int has_m, n_valid;
+    float cmax;
+    int pad1;
 };
 static_assert(sizeof(TileMeta) == 32, "TileMeta must be 32 bytes");
 
@@ -193,6 +197,8 @@ struct SumPlan {
 };
 SumPlan plan_sum(int mode, int precision, int n_src, int n_tgt);
 int set_sum_shape(int tpl, int want_wgs);   // 0, 0: built-in choice
+int set_ff_kernel(int kind);                // PYPO_FF_MATRIX or PYPO_FF_VECTOR
+int ff_kernel();
 size_t partial_bytes(int mode, const SumPlan &p, int n_tgt);
 
 // One PO propagation from packed sources to the target grid ct.  `res`

@@ -146,6 +146,8 @@ __host__ __device__ __forceinline__ int n_tiles(int n_src) { return (n_src + TIL
 // Launchers of the individual sum kernels (po_ff.hip, po_near.hip, po_scalar.hip).
 int launch_ff_sum(int precision, const SumPlan &p, const void *packed, int n_src, const GridPtrs &ct,
                   int n_tgt, double rev_scale, double *partial, hipStream_t st);
+int launch_ff_mfma(const SumPlan &p, const void *packed, int n_src, const GridPtrs &ct, int n_tgt,
+                   double rev_scale, double *partial, hipStream_t st);
 int launch_ff_finish(const SumPlan &p, const GridPtrs &ct, int n_tgt, double tdir, double pref,
                      const c2Bundle &out, const double *partial, hipStream_t st);
 int launch_near_sum(int mode, int precision, const SumPlan &p, const void *packed, int n_src,

@@ -260,6 +260,8 @@ __global__ __launch_bounds__(256) void k_ff_finish(const double *__restrict__ pa
 
 int launch_ff_sum(int precision, const SumPlan &p, const void *packed, int n_src, const GridPtrs &ct,
                   int n_tgt, double rev_scale, double *partial, hipStream_t st) {
+    if (precision != PYPO_PREC_FP64 && ff_kernel() == PYPO_FF_MATRIX)
+        return launch_ff_mfma(p, packed, n_src, ct, n_tgt, rev_scale, partial, st);
     dim3 grid(p.tblocks, p.splits), block(TILE);
     const int rb = precision == PYPO_PREC_FP64 ? (int)sizeof(SrcRec64) : (int)sizeof(SrcRec32);
     const TileMeta *meta = (const TileMeta *)((const char *)packed + meta_offset(rb, n_src));

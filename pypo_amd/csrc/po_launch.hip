@@ -9,7 +9,17 @@ namespace pypo {
 namespace {
 int g_tpl_override = 0;    // 0: built-in choice
 int g_wgs_override = 0;
+int g_ff_kernel = PYPO_FF_MATRIX;
 }  // namespace
+
+int set_ff_kernel(int kind) {
+    PYPO_REQUIRE(kind == PYPO_FF_MATRIX || kind == PYPO_FF_VECTOR, ERR_ARG,
+                 "far-field kernel: 0 (matrix cores) or 1 (vector ALUs)");
+    g_ff_kernel = kind;
+    return 0;
+}
+
+int ff_kernel() { return g_ff_kernel; }
 
 int set_sum_shape(int tpl, int want_wgs) {
     PYPO_REQUIRE(tpl >= 0 && tpl <= 2 && want_wgs >= 0 && want_wgs <= (1 << 20), ERR_ARG,

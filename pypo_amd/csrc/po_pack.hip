@@ -75,24 +75,33 @@ __global__ __launch_bounds__(256) void k_pack_s64(GridPtrs cs, int n, const doub
 }
 
 template <class Rec>
-__device__ __forceinline__ void rec_normal_m(const Rec &r, float &nx, float &ny, float &nz, bool &m);
+__device__ __forceinline__ void rec_normal_m(const Rec &r, float &nx, float &ny, float &nz, bool &m,
+                                             float &cmax);
 
 template <>
 __device__ __forceinline__ void rec_normal_m<SrcRec32>(const SrcRec32 &r, float &nx, float &ny,
-                                                       float &nz, bool &m) {
+                                                       float &nz, bool &m, float &cmax) {
     nx = r.nx; ny = r.ny; nz = r.nz;
     m = false;
+    cmax = 0.f;
 #pragma unroll
-    for (int q = 0; q < 6; q++) m |= (r.cm[q] != 0.f);
+    for (int q = 0; q < 6; q++) {
+        m |= (r.cm[q] != 0.f);
+        cmax = fmaxf(cmax, fmaxf(fabsf(r.cj[q]), fabsf(r.cm[q])));
+    }
 }
 
 template <>
 __device__ __forceinline__ void rec_normal_m<SrcRec64>(const SrcRec64 &r, float &nx, float &ny,
-                                                       float &nz, bool &m) {
+                                                       float &nz, bool &m, float &cmax) {
     nx = (float)r.nx; ny = (float)r.ny; nz = (float)r.nz;
     m = false;
+    cmax = 0.f;
 #pragma unroll
-    for (int q = 6; q < 12; q++) m |= (r.c[q] != 0.0);
+    for (int q = 0; q < 12; q++) {
+        if (q >= 6) m |= (r.c[q] != 0.0);
+        cmax = fmaxf(cmax, (float)fabs(r.c[q]));
+    }
 }
 
 // One workgroup per tile of 256 records.
@@ -102,18 +111,20 @@ __global__ __launch_bounds__(256) void k_tile_meta(const Rec *__restrict__ recs,
     __shared__ float scratch[4];
     const int i = blockIdx.x * TILE + threadIdx.x;
     const bool valid = i < n;
-    float nx = 0.f, ny = 0.f, nz = 1.f;
+    float nx = 0.f, ny = 0.f, nz = 1.f, cm = 0.f;
     bool m = false;
-    if (valid) rec_normal_m(recs[i], nx, ny, nz, m);
+    if (valid) rec_normal_m(recs[i], nx, ny, nz, m, cm);
     Cone c = block_cone(nx, ny, nz, valid, scratch);
     float any_m = block_sum(m ? 1.f : 0.f, scratch);
     float cnt = block_sum(valid ? 1.f : 0.f, scratch);
+    float cmax = -block_min(-cm, scratch);
     if (threadIdx.x == 0) {
         TileMeta t;
         t.ax = c.ax; t.ay = c.ay; t.az = c.az; t.cos_half = c.cos_half;
         t.has_m = any_m > 0.f ? 1 : 0;
         t.n_valid = (int)cnt;
-        t.pad0 = 0; t.pad1 = 0;
+        t.cmax = cmax;
+        t.pad1 = 0;
         meta[blockIdx.x] = t;
     }
 }

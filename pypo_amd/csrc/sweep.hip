@@ -296,7 +296,7 @@ __global__ __launch_bounds__(256) void k_sweep_meta(const PackInst *__restrict__
         t.ax = c.ax; t.ay = c.ay; t.az = c.az; t.cos_half = c.cos_half;
         t.has_m = any_m > 0.f ? 1 : 0;
         t.n_valid = (int)cnt;
-        t.pad0 = 0; t.pad1 = 0;
+        t.cmax = 0.f; t.pad1 = 0;
         in.meta[blockIdx.x] = t;
     }
 }

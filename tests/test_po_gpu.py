@@ -53,6 +53,57 @@ def test_po_matches_golden(hip, gold_po, name, prec):
     np.testing.assert_allclose(tg, gold_po[f"{name}/tgt_grid"], rtol=1e-13, atol=1e-12)
 
 
+@pytest.mark.parametrize("name", [n for n in sorted(C.po_cases()) if n.startswith("FF")])
+def test_ff_matrix_and_vector_kernels(hip, gold_po, name):
+    """The mixed far field on the matrix cores (FP16 hi/lo operands, FP32
+    accumulation) and on the vector ALUs (FP32 FMAs): both within the mixed
+    bar of the reference's output, and within 2e-6 of each other.  The
+    matrix kernel scales each tile's operands by a power of two, so scaling
+    the currents by 2^40 or 2^-40 scales its output exactly."""
+    c = C.po_cases()[name]
+    outs = {}
+    try:
+        for kind in ("vector", "matrix"):
+            B.set_ff_kernel(kind)
+            outs[kind] = flat(_run(c, "mixed")[0])
+            err = group_err(outs[kind], gold_po[f"{name}/out"])
+            print(f"{name} {kind}: {err:.2e}")
+            assert err <= TOL["mixed"]
+        assert group_err(outs["matrix"], outs["vector"]) <= 2e-6
+        cur = _src_for(c)
+        d = {"mode": "FF", "k": 2 * np.pi / c["lam"], "exp": c["exp"], "precision": "mixed"}
+        for f in (2.0 ** 40, 2.0 ** -40):
+            scaled = T.currents(*[f * comp for comp in cur.components()])
+            out = flat(B.PyPO_HIPd(c["source"], c["target"], dict(d, s_current=scaled)))
+            np.testing.assert_array_equal(out, f * outs["matrix"])
+    finally:
+        B.set_ff_kernel("matrix")
+
+
+def test_ff_matrix_kernel_edge_tiles(hip, port):
+    """Tiles that the matrix kernel treats specially: an all-zero tile (no
+    scale), a tile mixing magnitudes 1e-6..1 (small values below FP16's
+    normal range after scaling), magnetic currents, and a partial last tile
+    (775 = 3 * 256 + 7 sources), against the CPU checker."""
+    rng = np.random.default_rng(4)
+    src = G.plane({"gmode": "xy", "lims_x": [-20, 20], "lims_y": [-20, 20], "gridsize": [25, 31]})
+    n = 25 * 31                             # 3 full tiles and a partial one of 7
+    comps = [rng.normal(size=n) + 1j * rng.normal(size=n) for _ in range(6)]
+    mags = 10.0 ** rng.uniform(-6, 0, n)
+    comps = [cc * mags for cc in comps]
+    for cc in comps:
+        cc[:256] = 0.0                      # the first tile carries nothing
+    cur = T.currents(*[cc.reshape(25, 31) for cc in comps])
+    ff = G.plane({"gmode": "AoE", "lims_Az": [-30, 30], "lims_El": [-30, 30], "gridsize": [23, 19]})
+    k = 2 * np.pi / 3.0
+    B.set_ff_kernel("matrix")
+    out = B.PyPO_HIPd(src, ff, {"mode": "FF", "k": k, "s_current": cur, "precision": "mixed"})
+    ref, _, _ = OB.propagate(port, "FF", src, ff, cur, k)
+    err = group_err(flat(out), flat(ref))
+    print(f"edge tiles: {err:.2e}")
+    assert err <= 1e-5
+
+
 def _cfg1():
     """Plain PO configuration: vector Gaussian-like feed, lambda = 3 mm, on a
     101 x 101 plane over +-20 mm at the focus of a paraboloid (f = 0.5 m,
