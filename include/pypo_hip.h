@@ -384,6 +384,73 @@ int pypo_hip_power_sums_dev(const double *a_re, const double *a_im, const double
  * normalised mean direction, calcRMS (75-88) = sqrt(stats[7] / count). */
 int pypo_hip_frame_stats_dev(const cframe *fr, double *stats, void *stream);
 
+/* ---- Single-precision entry points of the reference's GPU library ----------
+ * The reference's GPU library (src/CUDA/InterfaceCUDA.h:29-69, loaded by
+ * BindGPU.loadGPUlib as libpypogpu) takes FP32 structs.  These wrappers keep
+ * its names and argument types: FP32 inputs are widened to FP64, the grids
+ * are generated and the sums run as in the FP64 entry points above (mixed
+ * precision by default, see pypo_hip_set_precision), and the results are
+ * narrowed to FP32.  The generated grids are written back to cs / ct like
+ * the reference's.  nBlocks and nThreads are launch hints and are ignored.
+ * Structs are byte-identical to Structs.h:216-362. */
+typedef struct arrC1f { float *x, *y; } arrC1f;
+typedef struct c2Bundlef {
+    float *r1x, *r1y, *r1z, *i1x, *i1y, *i1z;
+    float *r2x, *r2y, *r2z, *i2x, *i2y, *i2z;
+} c2Bundlef;
+typedef struct c4Bundlef {
+    float *r1x, *r1y, *r1z, *i1x, *i1y, *i1z;
+    float *r2x, *r2y, *r2z, *i2x, *i2y, *i2z;
+    float *r3x, *r3y, *r3z, *i3x, *i3y, *i3z;
+    float *r4x, *r4y, *r4z, *i4x, *i4y, *i4z;
+} c4Bundlef;
+typedef struct c2rBundlef {
+    float *r1x, *r1y, *r1z, *i1x, *i1y, *i1z;
+    float *r2x, *r2y, *r2z, *i2x, *i2y, *i2z;
+    float *r3x, *r3y, *r3z;
+} c2rBundlef;
+typedef struct reflparamsf {
+    float *coeffs;
+    float *lxu;
+    float *lyv;
+    int *n_cells;
+    bool flip;
+    int gmode;
+    float *gcenter;
+    float ecc_uv;
+    float rot_uv;
+    int type;
+    float *transf;
+    float rms;
+    unsigned int rms_seed;
+} reflparamsf;
+typedef struct reflcontainerf { int size; float *x, *y, *z, *nx, *ny, *nz, *area; } reflcontainerf;
+typedef struct cframef { int size; float *x, *y, *z, *dx, *dy, *dz; } cframef;
+
+/* InterfaceCUDA.h:31-59: PO into FP32 host bundles (t_direction -1 fwd, +1 bwd). */
+void callKernelf_JM(c2Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                    reflcontainerf *ct, c2Bundlef *currents, float k, float epsilon, float t_direction,
+                    int nBlocks, int nThreads);
+void callKernelf_EH(c2Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                    reflcontainerf *ct, c2Bundlef *currents, float k, float epsilon, float t_direction,
+                    int nBlocks, int nThreads);
+void callKernelf_JMEH(c4Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                      reflcontainerf *ct, c2Bundlef *currents, float k, float epsilon, float t_direction,
+                      int nBlocks, int nThreads);
+void callKernelf_EHP(c2rBundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                     reflcontainerf *ct, c2Bundlef *currents, float k, float epsilon, float t_direction,
+                     int nBlocks, int nThreads);
+void callKernelf_FF(c2Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                    reflcontainerf *ct, c2Bundlef *currents, float k, float epsilon, float t_direction,
+                    int nBlocks, int nThreads);
+/* InterfaceCUDA.h:61-64 */
+void callKernelf_scalar(arrC1f *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                        reflcontainerf *ct, arrC1f *inp, float k, float epsilon, float t_direction,
+                        int nBlocks, int nThreads);
+/* InterfaceCUDA.h:66-68: ray trace (epsilon = Newton tolerance). */
+void callRTKernel(reflparamsf ctp, cframef *fr_in, cframef *fr_out, float epsilon, float t0, int nBlocks,
+                  int nThreads);
+
 /* Mean kernel time (ms) of the last pypo_hip_propagate_dev / trace call's
  * dominant kernel, measured with HIP events on the launch stream, when
  * timing is enabled by pypo_hip_set_timing(1).  Returns -1 if unavailable. */

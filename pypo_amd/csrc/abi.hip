@@ -471,46 +471,32 @@ void propagateToFarField(c2Bundle *res, reflparams source, reflparams target, re
 
 // ---- drop-in ray trace (InterfaceCPU.h:60-61) -----------------------------------
 
-void propagateRays(reflparams ctp, cframe *fr_in, cframe *fr_out, int numThreads, double epsilon,
-                   double t0) {
-    (void)numThreads;
+// Host frames in and out, synchronous; returns 0 or the recorded error code.
+static int rays_dropin(const reflparams &ctp, cframe *fr_in, cframe *fr_out, double epsilon, double t0) {
     std::lock_guard<std::recursive_mutex> lock(g_mu);
     int err = ensure_init();
-    if (err) return;
-    if (!fr_in || !fr_out || fr_in->size != fr_out->size || fr_in->size < 0) {
-        set_error(ERR_ARG, "propagateRays: NULL frame or frame sizes differ");
-        return;
-    }
-    if (!ctp.coeffs || !ctp.transf) {
-        set_error(ERR_ARG, "propagateRays: reflparams has a NULL array");
-        return;
-    }
+    if (err) return err;
+    PYPO_REQUIRE(fr_in && fr_out && fr_in->size == fr_out->size && fr_in->size >= 0, ERR_ARG,
+                 "propagateRays: NULL frame or frame sizes differ");
+    PYPO_REQUIRE(ctp.coeffs && ctp.transf, ERR_ARG, "propagateRays: reflparams has a NULL array");
     const int n = fr_in->size;
-    if (n == 0) return;
+    if (n == 0) return 0;
     double *hin[6] = {fr_in->x, fr_in->y, fr_in->z, fr_in->dx, fr_in->dy, fr_in->dz};
     double *hout[6] = {fr_out->x, fr_out->y, fr_out->z, fr_out->dx, fr_out->dy, fr_out->dz};
-    for (int q = 0; q < 6; q++)
-        if (!hin[q] || !hout[q]) {
-            set_error(ERR_ARG, "propagateRays: frame has a NULL array");
-            return;
-        }
+    for (int q = 0; q < 6; q++) PYPO_REQUIRE(hin[q] && hout[q], ERR_ARG, "propagateRays: frame has a NULL array");
     hipStream_t st = g_stream;
-    if (begin_call(st)) return;
+    if ((err = begin_call(st))) return err;
     double *d = (double *)pool_get(SLOT_RAYS, 12 * (size_t)n * sizeof(double), &err);
-    if (err) return;
+    if (err) return err;
     for (int q = 0; q < 6; q++)
-        if (hipMemcpyAsync(d + q * (size_t)n, hin[q], n * sizeof(double), hipMemcpyHostToDevice, st) !=
-            hipSuccess) {
-            set_error(ERR_ALLOC, "propagateRays: upload failed");
-            return;
-        }
+        PYPO_HIP_TRY(hipMemcpyAsync(d + q * (size_t)n, hin[q], n * sizeof(double), hipMemcpyHostToDevice, st));
     const double *d_errs[1] = {nullptr};
     if (ctp.rms > 0) {
         std::vector<double> e = ray_deviates(ctp, n);
         double *de = (double *)pool_get(SLOT_RT_ERRS, n * sizeof(double), &err);
-        if (err) return;
-        (void)hipMemcpyAsync(de, e.data(), n * sizeof(double), hipMemcpyHostToDevice, st);
-        (void)hipStreamSynchronize(st);
+        if (err) return err;
+        PYPO_HIP_TRY(hipMemcpyAsync(de, e.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+        PYPO_HIP_TRY(hipStreamSynchronize(st));
         d_errs[0] = de;
     }
     cframe fr[2];
@@ -520,11 +506,152 @@ void propagateRays(reflparams ctp, cframe *fr_in, cframe *fr_out, int numThreads
         f0[q] = d + q * (size_t)n;
         f1[q] = d + (6 + q) * (size_t)n;
     }
-    if (launch_trace_chain(&ctp, d_errs, 1, fr, epsilon, t0, g_rt_max_iter, st, g_timing)) return;
+    if ((err = launch_trace_chain(&ctp, d_errs, 1, fr, epsilon, t0, g_rt_max_iter, st, g_timing))) return err;
     for (int q = 0; q < 6; q++)
-        (void)hipMemcpyAsync(hout[q], f1[q], n * sizeof(double), hipMemcpyDeviceToHost, st);
-    end_call(st);
-    if (hipStreamSynchronize(st) != hipSuccess) set_error(ERR_ALLOC, "propagateRays: sync failed");
+        PYPO_HIP_TRY(hipMemcpyAsync(hout[q], f1[q], n * sizeof(double), hipMemcpyDeviceToHost, st));
+    if ((err = end_call(st))) return err;
+    PYPO_HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+void propagateRays(reflparams ctp, cframe *fr_in, cframe *fr_out, int numThreads, double epsilon,
+                   double t0) {
+    (void)numThreads;
+    (void)rays_dropin(ctp, fr_in, fr_out, epsilon, t0);
+}
+
+// ---- single-precision drop-ins (InterfaceCUDA.h:29-69) ------------------------------
+
+// FP32 reflparams widened to FP64, with storage for the arrays it points to.
+struct WideRefl {
+    double coeffs[3], lxu[2], lyv[2], gcenter[2], transf[16];
+    int n_cells[2];
+    reflparams p;
+};
+
+static bool widen_refl(const reflparamsf &f, WideRefl &w, const char *what) {
+    if (!f.coeffs || !f.lxu || !f.lyv || !f.n_cells || !f.gcenter || !f.transf) {
+        set_error(ERR_ARG, std::string(what) + ": reflparamsf has a NULL array");
+        return false;
+    }
+    for (int i = 0; i < 3; i++) w.coeffs[i] = f.coeffs[i];
+    for (int i = 0; i < 2; i++) {
+        w.lxu[i] = f.lxu[i]; w.lyv[i] = f.lyv[i]; w.gcenter[i] = f.gcenter[i]; w.n_cells[i] = f.n_cells[i];
+    }
+    for (int i = 0; i < 16; i++) w.transf[i] = f.transf[i];
+    w.p.coeffs = w.coeffs; w.p.lxu = w.lxu; w.p.lyv = w.lyv; w.p.n_cells = w.n_cells;
+    w.p.flip = f.flip; w.p.gmode = f.gmode; w.p.gcenter = w.gcenter;
+    w.p.ecc_uv = f.ecc_uv; w.p.rot_uv = f.rot_uv; w.p.type = f.type;
+    w.p.transf = w.transf; w.p.rms = f.rms; w.p.rms_seed = f.rms_seed;
+    return true;
+}
+
+// n FP64 arrays of length len on the host, addressed as a pointer list.
+struct HostArrays {
+    std::vector<double> buf;
+    double *ptr[24];
+    HostArrays(int n, size_t len) : buf((size_t)n * len) {
+        for (int q = 0; q < n; q++) ptr[q] = buf.data() + q * len;
+    }
+};
+
+static bool float_arrays_ok(float *const *p, int n, const char *what) {
+    for (int q = 0; q < n; q++)
+        if (!p[q]) {
+            set_error(ERR_ARG, std::string(what) + ": FP32 bundle has a NULL array");
+            return false;
+        }
+    return true;
+}
+
+// FP32 PO call: widen, run po_dropin, narrow the result and the grids.
+static void po_float(int mode, void *resf, const reflparamsf &sf, const reflparamsf &tf, reflcontainerf *csf,
+                     reflcontainerf *ctf, const void *curf, float k, float epsilon, float tdir) {
+    if (!resf || !curf || !csf || !ctf) {
+        set_error(ERR_ARG, "callKernelf: NULL result, current or grid argument");
+        return;
+    }
+    const int gs = csf->size, gt = ctf->size;
+    if (gs <= 0 || gt <= 0) {
+        set_error(ERR_ARG, "callKernelf: empty source or target grid");
+        return;
+    }
+    const int nres = n_result_arrays(mode), ncur = mode == PYPO_MODE_SCALAR ? 2 : 12;
+    float *const *rf = (float *const *)resf;
+    float *const *cf = (float *const *)curf;
+    float *const gsf[7] = {csf->x, csf->y, csf->z, csf->nx, csf->ny, csf->nz, csf->area};
+    float *const gtf[7] = {ctf->x, ctf->y, ctf->z, ctf->nx, ctf->ny, ctf->nz, ctf->area};
+    if (!float_arrays_ok(rf, nres, "result") || !float_arrays_ok(cf, ncur, "currents") ||
+        !float_arrays_ok(gsf, 7, "source grid") || !float_arrays_ok(gtf, 7, "target grid"))
+        return;
+    WideRefl ws, wt;
+    if (!widen_refl(sf, ws, "source") || !widen_refl(tf, wt, "target")) return;
+    HostArrays cur(ncur, gs), res(nres, gt), gsd(7, gs), gtd(7, gt);
+    for (int q = 0; q < ncur; q++)
+        for (int i = 0; i < gs; i++) cur.ptr[q][i] = cf[q][i];
+    reflcontainer cs{gs, gsd.ptr[0], gsd.ptr[1], gsd.ptr[2], gsd.ptr[3], gsd.ptr[4], gsd.ptr[5], gsd.ptr[6]};
+    reflcontainer ct{gt, gtd.ptr[0], gtd.ptr[1], gtd.ptr[2], gtd.ptr[3], gtd.ptr[4], gtd.ptr[5], gtd.ptr[6]};
+    if (po_dropin(mode, res.ptr, ws.p, wt.p, &cs, &ct, cur.ptr, k, epsilon, tdir)) return;
+    for (int q = 0; q < nres; q++)
+        for (int i = 0; i < gt; i++) rf[q][i] = (float)res.ptr[q][i];
+    for (int q = 0; q < 7; q++) {
+        for (int i = 0; i < gs; i++) gsf[q][i] = (float)gsd.ptr[q][i];
+        for (int i = 0; i < gt; i++) gtf[q][i] = (float)gtd.ptr[q][i];
+    }
+}
+
+void callKernelf_JM(c2Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                    reflcontainerf *ct, c2Bundlef *currents, float k, float epsilon, float t_direction, int, int) {
+    po_float(PYPO_MODE_JM, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+void callKernelf_EH(c2Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                    reflcontainerf *ct, c2Bundlef *currents, float k, float epsilon, float t_direction, int, int) {
+    po_float(PYPO_MODE_EH, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+void callKernelf_JMEH(c4Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                      reflcontainerf *ct, c2Bundlef *currents, float k, float epsilon, float t_direction, int,
+                      int) {
+    po_float(PYPO_MODE_JMEH, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+void callKernelf_EHP(c2rBundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                     reflcontainerf *ct, c2Bundlef *currents, float k, float epsilon, float t_direction, int,
+                     int) {
+    po_float(PYPO_MODE_EHP, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+void callKernelf_FF(c2Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                    reflcontainerf *ct, c2Bundlef *currents, float k, float epsilon, float t_direction, int, int) {
+    po_float(PYPO_MODE_FF, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+void callKernelf_scalar(arrC1f *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                        reflcontainerf *ct, arrC1f *inp, float k, float epsilon, float t_direction, int, int) {
+    po_float(PYPO_MODE_SCALAR, res, source, target, cs, ct, inp, k, epsilon, t_direction);
+}
+
+void callRTKernel(reflparamsf ctp, cframef *fr_in, cframef *fr_out, float epsilon, float t0, int, int) {
+    if (!fr_in || !fr_out || fr_in->size != fr_out->size || fr_in->size < 0) {
+        set_error(ERR_ARG, "callRTKernel: NULL frame or frame sizes differ");
+        return;
+    }
+    const int n = fr_in->size;
+    float *const fi[6] = {fr_in->x, fr_in->y, fr_in->z, fr_in->dx, fr_in->dy, fr_in->dz};
+    float *const fo[6] = {fr_out->x, fr_out->y, fr_out->z, fr_out->dx, fr_out->dy, fr_out->dz};
+    if (!float_arrays_ok(fi, 6, "fr_in") || !float_arrays_ok(fo, 6, "fr_out")) return;
+    WideRefl w;
+    if (!widen_refl(ctp, w, "callRTKernel")) return;
+    if (n == 0) return;
+    HostArrays in(6, n), out(6, n);
+    for (int q = 0; q < 6; q++)
+        for (int i = 0; i < n; i++) in.ptr[q][i] = fi[q][i];
+    cframe a{n, in.ptr[0], in.ptr[1], in.ptr[2], in.ptr[3], in.ptr[4], in.ptr[5]};
+    cframe b{n, out.ptr[0], out.ptr[1], out.ptr[2], out.ptr[3], out.ptr[4], out.ptr[5]};
+    if (rays_dropin(w.p, &a, &b, epsilon, t0)) return;
+    for (int q = 0; q < 6; q++)
+        for (int i = 0; i < n; i++) fo[q][i] = (float)out.ptr[q][i];
 }
 
 // ---- drop-in grid generation (InterfaceReflector.h:21-22) ---------------------------

@@ -105,3 +105,42 @@ class aperparams(ctypes.Structure):
     0 ellipse, 1 rectangle, 2 none (every point)."""
     _fields_ = [("shape", ctypes.c_int), ("center", ctypes.c_double * 2), ("outer", ctypes.c_double * 2),
                 ("outer_x", ctypes.c_double * 2), ("outer_y", ctypes.c_double * 2)]
+
+
+# ---- FP32 twins (Structs.py:206-337), for the single-precision entry points ----
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def _ffields(names):
+    return [(n, _fp) for n in names]
+
+
+class arrC1f(ctypes.Structure):
+    _fields_ = _ffields(["x", "y"])
+
+
+class c2Bundlef(ctypes.Structure):
+    _fields_ = _ffields(_C2)
+
+
+class c4Bundlef(ctypes.Structure):
+    _fields_ = _ffields(_C4)
+
+
+class c2rBundlef(ctypes.Structure):
+    _fields_ = _ffields(_C2R)
+
+
+class reflparamsf(ctypes.Structure):
+    _fields_ = [("coeffs", _fp), ("lxu", _fp), ("lyv", _fp), ("n_cells", ctypes.POINTER(ctypes.c_int)),
+                ("flip", ctypes.c_bool), ("gmode", ctypes.c_int), ("gcenter", _fp), ("ecc_uv", ctypes.c_float),
+                ("rot_uv", ctypes.c_float), ("type", ctypes.c_int), ("transf", _fp), ("rms", ctypes.c_float),
+                ("rms_seed", ctypes.c_uint)]
+
+
+class reflcontainerf(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_int)] + _ffields(["x", "y", "z", "nx", "ny", "nz", "area"])
+
+
+class cframef(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_int)] + _ffields(["x", "y", "z", "dx", "dy", "dz"])

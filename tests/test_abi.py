@@ -83,3 +83,31 @@ def test_argument_errors_are_reported_not_fatal():
     assert lib.pypo_hip_last_error(buf, 256) == 0    # cleared after reading
     lib.pypo_hip_set_rt_max_iter(0)
     assert lib.pypo_hip_last_error(buf, 256) != 0
+
+
+def test_header_and_ctypes_layouts_agree(tmp_path):
+    """sizeof / offsetof of every struct in include/pypo_hip.h, compiled with
+    g++, against the ctypes twins (FP64 and the FP32 structs of Structs.h:216-362)."""
+    structs = ["arrC1", "c2Bundle", "c4Bundle", "c2rBundle", "reflparams", "reflcontainer", "cframe",
+               "arrC1f", "c2Bundlef", "c4Bundlef", "c2rBundlef", "reflparamsf", "reflcontainerf", "cframef",
+               "aperparams"]
+    lines = ["#include <cstdio>", "#include <cstddef>", f'#include "{HEADER}"', "int main() {"]
+    for s in structs:
+        lines.append(f'  printf("{s} SIZEOF %zu\\n", sizeof({s}));')
+        for name, _ in getattr(PS, s)._fields_:
+            lines.append(f'  printf("{s} {name} %zu\\n", offsetof({s}, {name}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.cpp"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["g++", "-std=c++11", str(src), "-o", str(exe)], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {(a, b): int(c) for a, b, c in (l.split() for l in out if l)}
+    for s in structs:
+        st = getattr(PS, s)
+        assert got[(s, "SIZEOF")] == ctypes.sizeof(st), s
+        for name, _ in st._fields_:
+            assert got[(s, name)] == getattr(st, name).offset, (s, name)
+    # FP32 reflparams as the reference's ctypes lays it out (Structs.py:293-310)
+    assert PS.reflparamsf.flip.offset == 32 and PS.reflparamsf.ecc_uv.offset == 48
+    assert PS.reflparamsf.transf.offset == 64 and ctypes.sizeof(PS.reflparamsf) == 80
