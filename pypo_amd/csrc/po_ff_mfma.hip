@@ -1,261 +1,11 @@
-// po_ff_mfma.hip -- the mixed-precision far-field sum on the matrix cores.
-//
-// The far field factors per target direction r into the source sums
-// (farfieldAtPoint, Propagation.h:1656-1781)
-//   S_q(r) = sum_s W(s, r) C_q(s),   W = exp(-j t k r_s.r),  C = w A (J, M),
-// a (targets x sources) by (sources x 12) complex matrix product whose left
-// factor is generated on the fly.  Each lane computes W for four sources and
-// one target -- FP64 phase in revolutions, v_fract_f64, FP32 v_sin/v_cos --
-// and the products run as v_mfma_f32_16x16x32_f16 with FP32 accumulation:
-//
-//   A (16 targets x 32) = [cos, sin] of 16 sources, interleaved along K;
-//   B (32 x 16 columns)  = per source s and output column q = 2a + part
-//                          (a = Jx..Mz, part 0 = real, 1 = imaginary):
-//                          cos row: Re C_a | Im C_a,  sin row: -Im C_a | Re C_a
-//                          (columns 12..15 zero).
-//
-// FP16 has 11 significant bits, so both operands are split into hi + lo
-// halves (x = hi + lo to ~22 bits) and three products are accumulated:
-// hi*hi + hi*lo + lo*hi (the lo*lo term is below FP32 rounding).  B is
-// scaled per 256-source tile by the power of two at or above the tile's
-// largest component (TileMeta::cmax), so it lies in [-1, 1]; the FP32 tile
-// sums are multiplied back and folded into FP64 after every tile, like the
-// vector kernel's.  The operand split costs three VALU instructions per pair
-// against the twelve FP32 FMAs the vector kernel spends on the products.
-//
-// Workgroup: 4 waves x MB M-blocks of 16 targets.  Per tile each lane takes
-// one FP32 source record from HBM (prefetched during the previous tile),
-// writes its coordinates to LDS (one array per coordinate: the four lane
-// groups of a wave then read distinct banks) and its B entries (16 K-steps
-// x 64 lanes x 16 B, hi and lo in separate arrays: 32 KB of LDS).  Lane layout of the
-// 16x16x32 f16 MFMA (checked on gfx950): lane l holds A[l % 16][8 (l / 16) + j],
-// B[8 (l / 16) + j][l % 16] (j = 0..7) and D[4 (l / 16) + i][l % 16] (i = 0..3).
+// po_ff_mfma.hip -- launchers of the matrix-core far-field sum (the kernel
+// body and its description are in po_ff_mfma.hpp): grid = target blocks of
+// 64 MB targets x source splits.
 
-#include "po_tiles.hpp"
+#include "po_ff_mfma.hpp"
 
 namespace pypo {
 namespace {
-
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef _Float16 half2v __attribute__((ext_vector_type(2)));
-typedef float floatx4 __attribute__((ext_vector_type(4)));
-
-constexpr int KSTEPS = TILE / 16;   // 16 sources (32 K rows: cos and sin) per MFMA
-
-// 1 / (power of two at or above cmax); 1 for an all-zero tile.
-__device__ __forceinline__ float tile_inv_scale(float cmax) {
-    if (!(cmax > 0.f)) return 1.f;
-    int e;
-    (void)frexpf(cmax, &e);   // cmax = m 2^e, m in [0.5, 1)
-    return ldexpf(1.f, -e);
-}
-
-__device__ __forceinline__ void split_pair(float c, float s, half2v &hi, half2v &lo) {
-    const _Float16 hc = (_Float16)c, hs = (_Float16)s;
-    hi = half2v{hc, hs};
-    lo = half2v{(_Float16)(c - (float)hc), (_Float16)(s - (float)hs)};
-}
-
-// The same split for the per-pair A operand: one packed RNE conversion for
-// hi, then lo = c - hi and s - hi rounded to FP16 by v_fma_mix{lo,hi}_f16
-// (FP32 c, s; hi read as FP16 halves; the FMA with -1 is exact before the
-// final rounding).  Measured on gfx950, conversions and the mixed forms issue
-// at about a quarter of the FP32 rate; this three-instruction form beat a
-// mask-and-subtract split with two packed conversions (114.4 vs 111.4 ms).
-__device__ __forceinline__ void split_pair_fast(float c, float s, unsigned &hi, unsigned &lo) {
-    hi = __builtin_bit_cast(unsigned, half2v{(_Float16)c, (_Float16)s});
-    asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(lo) : "v"(hi), "v"(c));
-    asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(lo) : "v"(hi), "v"(s));
-}
-
-// This lane's source record (tile slot sl) -> its B entries of every column.
-// Source slot sl sits in K-step sl / 16, lane group g = (sl % 16) / 4, K rows
-// 2 (sl % 4) (cos) and 2 (sl % 4) + 1 (sin) of the lanes 16 g + q.
-__device__ __forceinline__ void build_b(const SrcRec32 &r, int sl, float inv, half8 (*bhi)[64],
-                                        half8 (*blo)[64]) {
-    const int ks = sl >> 4, g = (sl & 15) >> 2, jj = (sl & 3);
-#pragma unroll
-    for (int a = 0; a < 6; a++) {
-        const float re = (a < 3 ? r.cj[2 * a] : r.cm[2 * (a - 3)]) * inv;
-        const float im = (a < 3 ? r.cj[2 * a + 1] : r.cm[2 * (a - 3) + 1]) * inv;
-        half2v h, l;
-        // real column 2a: cos row Re, sin row -Im
-        split_pair(re, -im, h, l);
-        reinterpret_cast<half2v *>(&bhi[ks][16 * g + 2 * a])[jj] = h;
-        reinterpret_cast<half2v *>(&blo[ks][16 * g + 2 * a])[jj] = l;
-        // imaginary column 2a+1: cos row Im, sin row Re
-        split_pair(im, re, h, l);
-        reinterpret_cast<half2v *>(&bhi[ks][16 * g + 2 * a + 1])[jj] = h;
-        reinterpret_cast<half2v *>(&blo[ks][16 * g + 2 * a + 1])[jj] = l;
-    }
-}
-
-// Source coordinates of a tile in LDS, one array per coordinate so that the
-// four lane groups of a wave (4 consecutive sources each) read distinct banks.
-struct TileXYZ {
-    double x[TILE], y[TILE], z[TILE];
-    float nx[TILE], ny[TILE], nz[TILE];
-};
-
-// One 256-source tile: 16 K-steps, each 3 MFMAs per M-block.  The MFMAs of
-// M-block m are issued between the pair computations of M-block m + 1
-// (sched_barrier pins the order): the matrix core runs while the wave
-// keeps issuing vector work.  Left to itself the compiler clusters the
-// three dependent MFMAs and the SIMD runs them and the vector work in turn.
-template <int MB, bool MASK>
-__device__ __forceinline__ void ff_tile_mfma(const TileXYZ &tx, const half8 (*bhi)[64], const half8 (*blo)[64],
-                                             int lane, const double (&qx)[MB], const double (&qy)[MB],
-                                             const double (&qz)[MB], const float (*tu)[64 * MB], int lt0,
-                                             floatx4 (&acc)[MB]) {
-    const int kb = lane >> 4;
-#pragma unroll 1
-    for (int ks = 0; ks < KSTEPS; ks++) {
-        const half8 b_hi = bhi[ks][lane], b_lo = blo[ks][lane];
-        const int s0 = ks * 16 + kb * 4;
-        double sx[4], sy[4], sz[4];
-        float nx[4], ny[4], nz[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            sx[j] = tx.x[s0 + j]; sy[j] = tx.y[s0 + j]; sz[j] = tx.z[s0 + j];
-            if (MASK) { nx[j] = tx.nx[s0 + j]; ny[j] = tx.ny[s0 + j]; nz[j] = tx.nz[s0 + j]; }
-        }
-        unsigned ah[2][4], al[2][4];
-        auto pair = [&](int m, int j, unsigned &h, unsigned &l) {
-            const double rev = __builtin_fma(sz[j], qz[m], __builtin_fma(sy[j], qy[m], sx[j] * qx[m]));
-            const float f = (float)__builtin_amdgcn_fract(rev);
-            float sn, cs;
-            sincos_rev_f32(f, sn, cs);
-            if (MASK) {
-                // target direction from LDS (kept out of registers: only masked tiles need it)
-                const int lt = lt0 + 16 * m;
-                const float d = __builtin_fmaf(nz[j], tu[2][lt], __builtin_fmaf(ny[j], tu[1][lt], nx[j] * tu[0][lt]));
-                cs = (d < 0.f) ? 0.f : cs;
-                sn = (d < 0.f) ? 0.f : sn;
-            }
-            split_pair_fast(cs, sn, h, l);
-        };
-#pragma unroll
-        for (int j = 0; j < 4; j++) pair(0, j, ah[0][j], al[0][j]);
-#pragma unroll
-        for (int m = 0; m < MB; m++) {
-            const int c = m & 1, n = c ^ 1;
-            const half8 a_hi = __builtin_bit_cast(half8, ah[c]), a_lo = __builtin_bit_cast(half8, al[c]);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if (j < 3) {
-                    __builtin_amdgcn_sched_barrier(0);
-                    if (j == 0) acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_hi, acc[m], 0, 0, 0);
-                    if (j == 1) acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_lo, acc[m], 0, 0, 0);
-                    if (j == 2) acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_lo, b_hi, acc[m], 0, 0, 0);
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-                if (m + 1 < MB) pair(m + 1, j, ah[n][j], al[n][j]);
-            }
-        }
-    }
-}
-
-template <int MB>
-__device__ __forceinline__ void ff_mfma_body(const SrcRec32 *__restrict__ src, const TileMeta *__restrict__ meta,
-                                             int n_src, int chunk, const double *__restrict__ th,
-                                             const double *__restrict__ ph, int n_tgt, double rev_scale,
-                                             double *__restrict__ partial) {
-    __shared__ TileXYZ tx;
-    __shared__ half8 bhi[KSTEPS][64], blo[KSTEPS][64];
-    __shared__ float tu[3][64 * MB];   // unit directions of the workgroup's targets
-    __shared__ float scratch[4];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int row = lane & 15, kb = lane >> 4;
-    const int t0 = blockIdx.x * (64 * MB) + wave * (16 * MB);   // first target of this wave
-
-    // the columns 12..15 of B stay zero
-    {
-        const int ks = tid >> 4, l = 16 * ((tid >> 2) & 3) + 12 + (tid & 3);
-        bhi[ks][l] = half8{};
-        blo[ks][l] = half8{};
-    }
-
-    // A-operand targets of this lane: t0 + 16 m + row
-    double qx[MB], qy[MB], qz[MB];
-    float fx[MB], fy[MB], fz[MB];
-    Cone cone;
-    {
-        bool any = false;
-#pragma unroll
-        for (int m = 0; m < MB; m++) {
-            const int t = t0 + 16 * m + row;
-            double x = 0, y = 0, z = 0;
-            if (t < n_tgt) {
-                double st, ct, sp, cp;
-                sincos(th[t], &st, &ct);
-                sincos(ph[t], &sp, &cp);
-                x = cp * st; y = sp * st; z = ct;
-            }
-            fx[m] = (float)x; fy[m] = (float)y; fz[m] = (float)z;
-            const int lt = wave * 16 * MB + 16 * m + row;
-            tu[0][lt] = fx[m]; tu[1][lt] = fy[m]; tu[2][lt] = fz[m];
-            qx[m] = x * rev_scale; qy[m] = y * rev_scale; qz[m] = z * rev_scale;
-            if (m == 0) any = t < n_tgt;
-        }
-        cone = block_cone(fx[0], fy[0], fz[0], any, scratch);
-        if (MB > 1 && cone.cos_half > 0.f) {
-            float d = 1.f;
-#pragma unroll
-            for (int m = 1; m < MB; m++)
-                if (t0 + 16 * m + row < n_tgt) d = fminf(d, fx[m] * cone.ax + fy[m] * cone.ay + fz[m] * cone.az);
-            cone.cos_half = fminf(cone.cos_half, block_min(d, scratch));
-        }
-    }
-
-    floatx4 acc[MB];
-    double tot[MB][4];
-#pragma unroll
-    for (int m = 0; m < MB; m++) {
-        acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int i = 0; i < 4; i++) tot[m][i] = 0.0;
-    }
-
-    const int s_begin = blockIdx.y * chunk;
-    const int s_end = min(n_src, s_begin + chunk);
-    const uint4 *g = reinterpret_cast<const uint4 *>(src);
-    TileLoader<6> ld;
-    ld.load(g, s_begin + tid, s_end);
-    for (int base = s_begin; base < s_end; base += TILE) {
-        const TileMeta tm = meta[base / TILE];
-        const float inv = tile_inv_scale(tm.cmax);
-        {
-            const SrcRec32 r = __builtin_bit_cast(SrcRec32, ld.buf);   // this lane's record
-            __syncthreads();
-            tx.x[tid] = r.x; tx.y[tid] = r.y; tx.z[tid] = r.z;
-            tx.nx[tid] = r.nx; tx.ny[tid] = r.ny; tx.nz[tid] = r.nz;
-            build_b(r, tid, inv, bhi, blo);
-            __syncthreads();
-        }
-        if (base + TILE < s_end) ld.load(g, base + TILE + tid, s_end);
-        if (cones_may_shadow(tm.ax, tm.ay, tm.az, tm.cos_half, cone))
-            ff_tile_mfma<MB, true>(tx, bhi, blo, lane, qx, qy, qz, tu, wave * 16 * MB + row, acc);
-        else
-            ff_tile_mfma<MB, false>(tx, bhi, blo, lane, qx, qy, qz, tu, wave * 16 * MB + row, acc);
-        const double scale = 1.0 / (double)inv;   // a power of two: exact
-#pragma unroll
-        for (int m = 0; m < MB; m++) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) tot[m][i] = __builtin_fma((double)acc[m][i], scale, tot[m][i]);
-            acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
-        }
-    }
-    // D layout: this lane holds column `row` of targets t0 + 16 m + 4 kb + i
-    if (row < 12) {
-#pragma unroll
-        for (int m = 0; m < MB; m++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int t = t0 + 16 * m + 4 * kb + i;
-                if (t < n_tgt) partial[((size_t)blockIdx.y * 12 + row) * n_tgt + t] = tot[m][i];
-            }
-    }
-}
 
 // Both shapes are held to two waves per SIMD (256 registers): one wave's
 // MFMAs then overlap the other's vector work.  8 M-blocks amortise the
@@ -264,14 +14,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const SrcRec32 *__restrict__ src, const TileMeta *__restrict__ meta, int n_src, int chunk,
     const double *__restrict__ th, const double *__restrict__ ph, int n_tgt, double rev_scale,
     double *__restrict__ partial) {
-    ff_mfma_body<4>(src, meta, n_src, chunk, th, ph, n_tgt, rev_scale, partial);
+    const int s0 = blockIdx.y * chunk;
+    mfma::ff_mfma_body<4>(src, meta, s0, min(n_src, s0 + chunk), blockIdx.x * 256, th, ph, n_tgt, rev_scale,
+                          partial + (size_t)blockIdx.y * 12 * n_tgt);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_ff_mfma8(const SrcRec32 *__restrict__ src, const TileMeta *__restrict__ meta,
-                                                  int n_src, int chunk, const double *__restrict__ th,
-                                                  const double *__restrict__ ph, int n_tgt, double rev_scale,
-                                                  double *__restrict__ partial) {
-    ff_mfma_body<8>(src, meta, n_src, chunk, th, ph, n_tgt, rev_scale, partial);
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_ff_mfma8(
+    const SrcRec32 *__restrict__ src, const TileMeta *__restrict__ meta, int n_src, int chunk,
+    const double *__restrict__ th, const double *__restrict__ ph, int n_tgt, double rev_scale,
+    double *__restrict__ partial) {
+    const int s0 = blockIdx.y * chunk;
+    mfma::ff_mfma_body<8>(src, meta, s0, min(n_src, s0 + chunk), blockIdx.x * 512, th, ph, n_tgt, rev_scale,
+                          partial + (size_t)blockIdx.y * 12 * n_tgt);
 }
 
 }  // namespace

@@ -18,7 +18,7 @@
 #include <cstring>
 #include <vector>
 
-#include "po_tiles.hpp"
+#include "po_ff_mfma.hpp"
 
 namespace pypo {
 
@@ -50,6 +50,20 @@ struct PackInst {
 };
 
 namespace {
+
+// The far-field stage on the matrix cores (po_ff_mfma.hpp): one work item at
+// a time, 64 MB targets per item (MB = 4 TPL, matching the item's target block).
+template <int MB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sweep_ff_mfma(
+    const StageInst *__restrict__ insts, const SweepItem *__restrict__ items, int n_items) {
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const SweepItem item = items[it];
+        const StageInst &in = insts[item.inst];
+        mfma::ff_mfma_body<MB>(in.src, in.meta, item.s0, item.s1, item.tblock * 64 * MB, in.tx, in.ty, in.n_tgt,
+                               in.rev_scale, in.partial + (size_t)item.chunk * 12 * in.n_tgt);
+        __syncthreads();   // the next item reuses the LDS tiles
+    }
+}
 
 template <int TPL>
 __global__ __launch_bounds__(256) void k_sweep_ff(const StageInst *__restrict__ insts,
@@ -280,23 +294,27 @@ __global__ __launch_bounds__(256) void k_sweep_meta(const PackInst *__restrict__
     if ((int)blockIdx.x * TILE >= in.n) return;   // uniform per workgroup
     const int i = blockIdx.x * TILE + threadIdx.x;
     const bool valid = i < in.n;
-    float nx = 0.f, ny = 0.f, nz = 1.f;
+    float nx = 0.f, ny = 0.f, nz = 1.f, cm = 0.f;
     bool m = false;
     if (valid) {
         const SrcRec32 &r = in.rec[i];
         nx = r.nx; ny = r.ny; nz = r.nz;
 #pragma unroll
-        for (int q = 0; q < 6; q++) m |= (r.cm[q] != 0.f);
+        for (int q = 0; q < 6; q++) {
+            m |= (r.cm[q] != 0.f);
+            cm = fmaxf(cm, fmaxf(fabsf(r.cj[q]), fabsf(r.cm[q])));
+        }
     }
     Cone c = block_cone(nx, ny, nz, valid, scratch);
     float any_m = block_sum(m ? 1.f : 0.f, scratch);
     float cnt = block_sum(valid ? 1.f : 0.f, scratch);
+    float cmax = -block_min(-cm, scratch);
     if (threadIdx.x == 0) {
         TileMeta t;
         t.ax = c.ax; t.ay = c.ay; t.az = c.az; t.cos_half = c.cos_half;
         t.has_m = any_m > 0.f ? 1 : 0;
         t.n_valid = (int)cnt;
-        t.cmax = 0.f; t.pad1 = 0;
+        t.cmax = cmax; t.pad1 = 0;
         in.meta[blockIdx.x] = t;
     }
 }
@@ -387,8 +405,14 @@ int launch_stage_sum(bool far, const Sweep *s, hipStream_t st) {
     const int n = far ? s->n_items_ff : s->n_items_jm;
     if (n == 0) return 0;
     if (far) {
-        if (s->prm.tpl == 2) hipLaunchKernelGGL(k_sweep_ff<2>, dim3(grid), dim3(TILE), 0, st, ins, items, n);
-        else hipLaunchKernelGGL(k_sweep_ff<1>, dim3(grid), dim3(TILE), 0, st, ins, items, n);
+        if (ff_kernel() == PYPO_FF_MATRIX) {
+            if (s->prm.tpl == 2) hipLaunchKernelGGL(k_sweep_ff_mfma<8>, dim3(grid), dim3(TILE), 0, st, ins, items, n);
+            else hipLaunchKernelGGL(k_sweep_ff_mfma<4>, dim3(grid), dim3(TILE), 0, st, ins, items, n);
+        } else if (s->prm.tpl == 2) {
+            hipLaunchKernelGGL(k_sweep_ff<2>, dim3(grid), dim3(TILE), 0, st, ins, items, n);
+        } else {
+            hipLaunchKernelGGL(k_sweep_ff<1>, dim3(grid), dim3(TILE), 0, st, ins, items, n);
+        }
     } else {
         // the near-field work list is built with one target per lane
         hipLaunchKernelGGL(k_sweep_near<1>, dim3(grid), dim3(TILE), 0, st, ins, items, n);
