@@ -17,7 +17,7 @@
 // The shadow tests of the forward sums are applied here too; their own
 // derivatives are zero almost everywhere.
 
-#include "po_common.hpp"
+#include "po_ff_mfma.hpp"
 
 #include <algorithm>
 
@@ -147,6 +147,216 @@ __global__ __launch_bounds__(256) void k_adj_reverse32(const TgtRec32 *__restric
 #pragma unroll
             for (int q = 0; q < 12; q++) partial[((size_t)blockIdx.y * 12 + q) * n_src + i] = tot[j][q];
         }
+    }
+}
+
+// ---- reverse sum on the matrix cores ----------------------------------------------
+// B_i and G_i = sum_a n_i,a G_i,a with G_i,a = sum_t r_t,a A_t W(i, t) form a
+// (dish points x targets) by (targets x 24) complex product: rows are dish
+// points (16 per M-block), K runs over targets (cos and sin rows), and the
+// 24 real columns (A, r_x A | r_y A, r_z A) fill two 16-column blocks.  The
+// phase, FP32 sin/cos, the FP16 hi/lo operand split and the per-tile operand
+// scale are those of the far-field kernel (po_ff_mfma.hpp); the shadow test
+// n_i.r_t >= 0 zeroes W where a cone test cannot exclude it.  G_i is formed
+// from the three G_i,a in FP64 by k_adj_combine24.
+
+// Per 256-target tile: the cone of the directions r_t and the largest |A_t|
+// component (the B operand's scale).
+__global__ __launch_bounds__(256) void k_adj_tgt_meta(const TgtRec32 *__restrict__ tg, int n_tgt,
+                                                      TileMeta *__restrict__ meta) {
+    __shared__ float scratch[4];
+    const int t = blockIdx.x * TILE + threadIdx.x;
+    const bool valid = t < n_tgt;
+    float rx = 0.f, ry = 0.f, rz = 1.f, am = 0.f;
+    if (valid) {
+        const TgtRec32 &r = tg[t];
+        rx = r.rx; ry = r.ry; rz = r.rz;
+#pragma unroll
+        for (int q = 0; q < 6; q++) am = fmaxf(am, fabsf(r.a[q]));
+    }
+    const Cone c = block_cone(rx, ry, rz, valid, scratch);
+    const float amax = -block_min(-am, scratch);
+    if (threadIdx.x == 0) {
+        TileMeta m{};
+        m.ax = c.ax; m.ay = c.ay; m.az = c.az; m.cos_half = c.cos_half;
+        m.n_valid = min(TILE, n_tgt - (int)blockIdx.x * TILE);
+        m.cmax = amax;
+        meta[blockIdx.x] = m;
+    }
+}
+
+struct RevTile {
+    double u[3][TILE];       // r_t * rev_scale
+    float r[3][TILE];        // r_t (shadow test)
+};
+
+// This lane's target record (tile slot sl) -> its entries of both B blocks.
+__device__ __forceinline__ void build_rev_b(const TgtRec32 &r, int sl, float inv, mfma::half8 (*b0h)[64],
+                                            mfma::half8 (*b0l)[64], mfma::half8 (*b1h)[64],
+                                            mfma::half8 (*b1l)[64]) {
+    using mfma::half2v;
+    const int ks = sl >> 4, g = (sl & 15) >> 2, jj = sl & 3;
+    const float w[4] = {1.f, r.rx, r.ry, r.rz};
+#pragma unroll
+    for (int v = 0; v < 4; v++) {   // A, r_x A, r_y A, r_z A
+        mfma::half8 (*bh)[64] = v < 2 ? b0h : b1h;
+        mfma::half8 (*bl)[64] = v < 2 ? b0l : b1l;
+        const int c0 = (v & 1) * 6;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const float re = r.a[2 * q] * w[v] * inv, im = r.a[2 * q + 1] * w[v] * inv;
+            half2v h, l;
+            mfma::split_pair(re, -im, h, l);
+            reinterpret_cast<half2v *>(&bh[ks][16 * g + c0 + 2 * q])[jj] = h;
+            reinterpret_cast<half2v *>(&bl[ks][16 * g + c0 + 2 * q])[jj] = l;
+            mfma::split_pair(im, re, h, l);
+            reinterpret_cast<half2v *>(&bh[ks][16 * g + c0 + 2 * q + 1])[jj] = h;
+            reinterpret_cast<half2v *>(&bl[ks][16 * g + c0 + 2 * q + 1])[jj] = l;
+        }
+    }
+}
+
+template <int MB, bool MASK>
+__device__ __forceinline__ void rev_tile_mfma(const RevTile &tt, const mfma::half8 (*b0h)[64],
+                                              const mfma::half8 (*b0l)[64], const mfma::half8 (*b1h)[64],
+                                              const mfma::half8 (*b1l)[64], int lane, const double (&px)[MB],
+                                              const double (&py)[MB], const double (&pz)[MB],
+                                              const float (*nv)[64 * MB], int lt0, mfma::floatx4 (&acc0)[MB],
+                                              mfma::floatx4 (&acc1)[MB]) {
+    using mfma::half8;
+    const int kb = lane >> 4;
+#pragma unroll 1
+    for (int ks = 0; ks < mfma::KSTEPS; ks++) {
+        const half8 h0 = b0h[ks][lane], l0 = b0l[ks][lane], h1 = b1h[ks][lane], l1 = b1l[ks][lane];
+        const int t0 = ks * 16 + kb * 4;
+        double ux[4], uy[4], uz[4];
+        float rx[4], ry[4], rz[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            ux[j] = tt.u[0][t0 + j]; uy[j] = tt.u[1][t0 + j]; uz[j] = tt.u[2][t0 + j];
+            if (MASK) { rx[j] = tt.r[0][t0 + j]; ry[j] = tt.r[1][t0 + j]; rz[j] = tt.r[2][t0 + j]; }
+        }
+#pragma unroll
+        for (int m = 0; m < MB; m++) {
+            unsigned ah[4], al[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const double rev = __builtin_fma(pz[m], uz[j], __builtin_fma(py[m], uy[j], px[m] * ux[j]));
+                const float f = (float)__builtin_amdgcn_fract(rev);
+                float sn, cs;
+                sincos_rev_f32(f, sn, cs);
+                if (MASK) {
+                    const int lt = lt0 + 16 * m;
+                    const float d = __builtin_fmaf(nv[2][lt], rz[j], __builtin_fmaf(nv[1][lt], ry[j], nv[0][lt] * rx[j]));
+                    cs = (d < 0.f) ? 0.f : cs;
+                    sn = (d < 0.f) ? 0.f : sn;
+                }
+                mfma::split_pair_fast(cs, sn, ah[j], al[j]);
+            }
+            const half8 a_hi = __builtin_bit_cast(half8, ah), a_lo = __builtin_bit_cast(half8, al);
+            acc0[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, h0, acc0[m], 0, 0, 0);
+            acc1[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, h1, acc1[m], 0, 0, 0);
+            acc0[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, l0, acc0[m], 0, 0, 0);
+            acc1[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, l1, acc1[m], 0, 0, 0);
+            acc0[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_lo, h0, acc0[m], 0, 0, 0);
+            acc1[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_lo, h1, acc1[m], 0, 0, 0);
+        }
+    }
+}
+
+// Partial layout [split][24][n_src]: q 0..5 = B, 6..11 = G_x, 12..17 = G_y,
+// 18..23 = G_z (re/im xyz interleaved).  Grid: (dish blocks of 64 MB points,
+// target splits).
+template <int MB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_adj_reverse_mfma(
+    const TgtRec32 *__restrict__ tg, const TileMeta *__restrict__ tmeta, int n_tgt, int chunk, GridPtrs src,
+    int n_src, double *__restrict__ partial) {
+    using mfma::floatx4;
+    using mfma::half8;
+    __shared__ RevTile tt;
+    __shared__ half8 b0h[mfma::KSTEPS][64], b0l[mfma::KSTEPS][64], b1h[mfma::KSTEPS][64], b1l[mfma::KSTEPS][64];
+    __shared__ float nv[3][64 * MB];   // normals of the workgroup's dish points (shadow test)
+    __shared__ float scratch[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int row = lane & 15, kb = lane >> 4;
+    const int s0 = blockIdx.x * (64 * MB) + wave * (16 * MB);
+    {
+        const int ks = tid >> 4, l = 16 * ((tid >> 2) & 3) + 12 + (tid & 3);
+        b0h[ks][l] = half8{}; b0l[ks][l] = half8{}; b1h[ks][l] = half8{}; b1l[ks][l] = half8{};
+    }
+    double px[MB], py[MB], pz[MB];
+    Cone cone;
+    {
+        float fx[MB], fy[MB], fz[MB];
+        bool any = false;
+#pragma unroll
+        for (int m = 0; m < MB; m++) {
+            const int i = s0 + 16 * m + row;
+            const bool valid = i < n_src;
+            px[m] = valid ? src.x[i] : 0.0; py[m] = valid ? src.y[i] : 0.0; pz[m] = valid ? src.z[i] : 0.0;
+            fx[m] = valid ? (float)src.nx[i] : 0.f; fy[m] = valid ? (float)src.ny[i] : 0.f;
+            fz[m] = valid ? (float)src.nz[i] : 1.f;
+            const int lt = wave * 16 * MB + 16 * m + row;
+            nv[0][lt] = fx[m]; nv[1][lt] = fy[m]; nv[2][lt] = fz[m];
+            if (m == 0) any = valid;
+        }
+        cone = block_cone(fx[0], fy[0], fz[0], any, scratch);
+        if (MB > 1 && cone.cos_half > 0.f) {
+            float d = 1.f;
+#pragma unroll
+            for (int m = 1; m < MB; m++)
+                if (s0 + 16 * m + row < n_src) d = fminf(d, fx[m] * cone.ax + fy[m] * cone.ay + fz[m] * cone.az);
+            cone.cos_half = fminf(cone.cos_half, block_min(d, scratch));
+        }
+    }
+    floatx4 acc0[MB], acc1[MB];
+    double tot0[MB][4], tot1[MB][4];
+#pragma unroll
+    for (int m = 0; m < MB; m++) {
+        acc0[m] = floatx4{0.f, 0.f, 0.f, 0.f};
+        acc1[m] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int i = 0; i < 4; i++) { tot0[m][i] = 0.0; tot1[m][i] = 0.0; }
+    }
+    const int t_begin = blockIdx.y * chunk;
+    const int t_end = min(n_tgt, t_begin + chunk);
+    for (int base = t_begin; base < t_end; base += TILE) {
+        const TileMeta tm = tmeta[base / TILE];
+        const float inv = mfma::tile_inv_scale(tm.cmax);
+        TgtRec32 r{};
+        if (base + tid < t_end) r = tg[base + tid];
+        __syncthreads();
+        tt.u[0][tid] = r.ux; tt.u[1][tid] = r.uy; tt.u[2][tid] = r.uz;
+        tt.r[0][tid] = r.rx; tt.r[1][tid] = r.ry; tt.r[2][tid] = r.rz;
+        build_rev_b(r, tid, inv, b0h, b0l, b1h, b1l);
+        __syncthreads();
+        if (cones_may_shadow(tm.ax, tm.ay, tm.az, tm.cos_half, cone))
+            rev_tile_mfma<MB, true>(tt, b0h, b0l, b1h, b1l, lane, px, py, pz, nv, wave * 16 * MB + row, acc0, acc1);
+        else
+            rev_tile_mfma<MB, false>(tt, b0h, b0l, b1h, b1l, lane, px, py, pz, nv, wave * 16 * MB + row, acc0, acc1);
+        const double scale = 1.0 / (double)inv;
+#pragma unroll
+        for (int m = 0; m < MB; m++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                tot0[m][i] = __builtin_fma((double)acc0[m][i], scale, tot0[m][i]);
+                tot1[m][i] = __builtin_fma((double)acc1[m][i], scale, tot1[m][i]);
+            }
+            acc0[m] = floatx4{0.f, 0.f, 0.f, 0.f};
+            acc1[m] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+    if (row < 12) {
+#pragma unroll
+        for (int m = 0; m < MB; m++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int p = s0 + 16 * m + 4 * kb + i;
+                if (p < n_src) {
+                    partial[((size_t)blockIdx.y * 24 + row) * n_src + p] = tot0[m][i];
+                    partial[((size_t)blockIdx.y * 24 + 12 + row) * n_src + p] = tot1[m][i];
+                }
+            }
     }
 }
 
@@ -445,6 +655,40 @@ __global__ __launch_bounds__(256) void k_adj_combine(const double *__restrict__ 
     grad[i] = c * (re_bdj + tdir * k * im_jg);
 }
 
+// The 24-row layout of k_adj_reverse_mfma: G = n_x G_x + n_y G_y + n_z G_z in FP64.
+__global__ __launch_bounds__(256) void k_adj_combine24(const double *__restrict__ partial, int splits,
+                                                       GridPtrs dish, int n, int gmode, int ncx, int ncy,
+                                                       const double *__restrict__ jd, double k, double tdir,
+                                                       double *__restrict__ grad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double S[24];
+#pragma unroll
+    for (int q = 0; q < 24; q++) {
+        double v = 0.0;
+        for (int s = 0; s < splits; s++) v += partial[((size_t)s * 24 + q) * n + i];
+        S[q] = v;
+    }
+    const double nrm[3] = {dish.nx[i], dish.ny[i], dish.nz[i]};
+    double re_bdj = 0.0, im_jg = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double br = S[2 * a], bi = S[2 * a + 1];
+        double gr = 0.0, gi = 0.0;
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            gr += nrm[b] * S[6 + 6 * b + 2 * a];
+            gi += nrm[b] * S[7 + 6 * b + 2 * a];
+        }
+        const double jr = jd[(size_t)a * n + i], ji = jd[(size_t)(3 + a) * n + i];
+        const double dr = jd[(size_t)(6 + a) * n + i], di = jd[(size_t)(9 + a) * n + i];
+        re_bdj += br * dr - bi * di;
+        im_jg += jr * gi + ji * gr;
+    }
+    const double c = trapezoid_weight(i, gmode, ncx, ncy) * dish.area[i];
+    grad[i] = c * (re_bdj + tdir * k * im_jg);
+}
+
 __global__ __launch_bounds__(256) void k_displace(GridPtrs g0, const double *__restrict__ d, int n,
                                                   GridPtrs out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -473,6 +717,8 @@ namespace {
 // Target splits of the reverse sum: enough workgroups to fill the chip,
 // at least four 256-target tiles per split.
 struct AdjPlan {
+    bool matrix;   // reverse sum on the matrix cores (mixed precision)
+    int rows;      // partial rows per split: 24 (matrix) or 12
     int spl;       // dish points per lane in the reverse sum
     int blocks;    // workgroups of one point per lane
     int rblocks;   // reverse-sum workgroups along the dish points
@@ -482,18 +728,22 @@ struct AdjPlan {
 
 AdjPlan adj_plan(int precision, int n_feed, int n_dish, int n_tgt) {
     AdjPlan p;
+    p.matrix = precision != PYPO_PREC_FP64 && ff_kernel() == PYPO_FF_MATRIX;
+    p.rows = p.matrix ? 24 : 12;
     p.spl = precision == PYPO_PREC_FP64 ? 1 : 2;
     p.blocks = (n_dish + TILE - 1) / TILE;
-    p.rblocks = (n_dish + TILE * p.spl - 1) / (TILE * p.spl);
+    // matrix reverse sum: 4 M-blocks of 16 dish points per wave, 256 per workgroup
+    p.rblocks = p.matrix ? (n_dish + TILE - 1) / TILE : (n_dish + TILE * p.spl - 1) / (TILE * p.spl);
     int splits = (2048 + p.rblocks - 1) / p.rblocks;
     const int max_splits = (n_tgt + 4 * TILE - 1) / (4 * TILE);
     splits = std::max(1, std::min(splits, max_splits));
     p.chunk = ((n_tgt + splits - 1) / splits + TILE - 1) / TILE * TILE;
     p.splits = (n_tgt + p.chunk - 1) / p.chunk;
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-    p.tg_bytes = up((size_t)n_tgt * (precision == PYPO_PREC_FP64 ? sizeof(TgtRec64) : sizeof(TgtRec32)));
+    p.tg_bytes = up((size_t)n_tgt * (precision == PYPO_PREC_FP64 ? sizeof(TgtRec64) : sizeof(TgtRec32))) +
+                 up((size_t)((n_tgt + TILE - 1) / TILE) * sizeof(TileMeta));
     p.feed_bytes = up(packed_bytes(PYPO_MODE_JM, precision, n_feed));
-    p.part_bytes = up((size_t)p.splits * 12 * n_dish * sizeof(double));
+    p.part_bytes = up((size_t)p.splits * p.rows * n_dish * sizeof(double));
     p.jd_bytes = up((size_t)12 * n_dish * sizeof(double));
     return p;
 }
@@ -542,15 +792,27 @@ int launch_ff_power_grad(int precision, const GridPtrs &dish, int n_dish, int gm
         hipLaunchKernelGGL(k_adj_currents32, dim3(blocks), dim3(TILE), 0, st, (const SrcRec32 *)fpk,
                            n_feed, dish, n_dish, k, tdir, pc.prefactor, jd);
     PYPO_CHECK_LAUNCH();
-    if (fp64)
+    if (fp64) {
         hipLaunchKernelGGL(k_adj_reverse64, dim3(p.rblocks, splits), dim3(TILE), 0, st,
                            (const TgtRec64 *)tg, n_tgt, chunk, dish, n_dish, part);
-    else
+    } else if (p.matrix) {
+        TileMeta *tmeta = (TileMeta *)((char *)tg + (((size_t)n_tgt * sizeof(TgtRec32) + 255) / 256 * 256));
+        hipLaunchKernelGGL(k_adj_tgt_meta, dim3((n_tgt + TILE - 1) / TILE), dim3(TILE), 0, st,
+                           (const TgtRec32 *)tg, n_tgt, tmeta);
+        PYPO_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_adj_reverse_mfma<4>, dim3(p.rblocks, splits), dim3(TILE), 0, st,
+                           (const TgtRec32 *)tg, (const TileMeta *)tmeta, n_tgt, chunk, dish, n_dish, part);
+    } else {
         hipLaunchKernelGGL(k_adj_reverse32<2>, dim3(p.rblocks, splits), dim3(TILE), 0, st,
                            (const TgtRec32 *)tg, n_tgt, chunk, dish, n_dish, part);
+    }
     PYPO_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_adj_combine, dim3(blocks), dim3(256), 0, st, part, splits, dish, n_dish, gmode,
-                       ncx, ncy, jd, k, tdir, grad);
+    if (p.matrix)
+        hipLaunchKernelGGL(k_adj_combine24, dim3(blocks), dim3(256), 0, st, part, splits, dish, n_dish, gmode,
+                           ncx, ncy, jd, k, tdir, grad);
+    else
+        hipLaunchKernelGGL(k_adj_combine, dim3(blocks), dim3(256), 0, st, part, splits, dish, n_dish, gmode,
+                           ncx, ncy, jd, k, tdir, grad);
     PYPO_CHECK_LAUNCH();
     return 0;
 }

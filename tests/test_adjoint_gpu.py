@@ -72,6 +72,35 @@ def test_gradient_matches_checker(small, precision, tol):
     assert rel <= tol and np.abs(got - ref).max() <= tol * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("ff_deg", [6.0, 70.0])
+def test_gradient_matrix_and_vector_reverse_sums(hip, port, ff_deg):
+    """The mixed backward far-field sum on the matrix cores (24 output columns
+    B, r_x A, r_y A, r_z A; G recombined with the normals in FP64) and on the
+    vector ALUs, against the checker; the +-70 deg far field puts pairs in
+    the shadow test, which the cone test cannot skip there."""
+    from pypo_amd import bind as B
+    c = small_shape_case(port, ff_deg=ff_deg)
+    prob = PA.ShapeProblem(c["feed"], c["cur"], c["dish"], c["ff"], c["k"])
+    d = torch.tensor(c["d"], dtype=torch.float64, device="cuda")
+    E, _ = OA.forward(port, c["fr"], c["fcells"], c["fc"], c["dr"], c["dcells"], c["d"], c["ffr"], c["k"])
+    g = np.random.default_rng(7).standard_normal(E.shape[1]) / np.abs(E).max() ** 2
+    ref = OA.power_grad(c["fr"], c["fcells"], c["fc"], OA.displace(c["dr"], c["d"]), c["dcells"], c["ffr"],
+                        E, g, c["k"])
+    grads = {}
+    try:
+        for kind in ("vector", "matrix"):
+            B.set_ff_kernel(kind)
+            got = prob.power_grad(d, torch.tensor(E, device="cuda"), torch.tensor(g, device="cuda"),
+                                  precision="mixed").cpu().numpy()
+            rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            print(f"ff +-{ff_deg} deg, {kind}: gradient error {rel:.2e}")
+            assert rel <= 1e-5 and np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+            grads[kind] = got
+    finally:
+        B.set_ff_kernel("matrix")
+    assert np.linalg.norm(grads["matrix"] - grads["vector"]) <= 2e-6 * np.linalg.norm(ref)
+
+
 def test_autograd_function_matches_explicit_call(small):
     c, prob, d0 = small
     T = torch.tensor(c["T"], device="cuda")
