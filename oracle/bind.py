@@ -1,4 +1,4 @@
-"""TEST INFRASTRUCTURE ONLY: ctypes driver for the CPU checkers.
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the CPU checkers.
 
   kind="port" -> oracle/liboracle.so, this project's FP64 restatement of the
                  reference CPU path (pypo_oracle.cpp), symbols "oracle_*".

@@ -2,7 +2,7 @@
 
 Bars: FP64 path <= 1e-10 of each field's peak; mixed path (FP32 integrand,
 FP64 distance/phase) <= 1e-5 of each field's peak -- the tolerance the
-north-star metric is quoted at.  Peaks are taken per field (E, H, J, M,
+headline far-field benchmark is quoted at.  Peaks are taken per field (E, H, J, M,
 S_r separately).
 """
 
