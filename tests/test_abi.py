@@ -111,3 +111,44 @@ def test_header_and_ctypes_layouts_agree(tmp_path):
     # FP32 reflparams as the reference's ctypes lays it out (Structs.py:293-310)
     assert PS.reflparamsf.flip.offset == 32 and PS.reflparamsf.ecc_uv.offset == 48
     assert PS.reflparamsf.transf.offset == 64 and ctypes.sizeof(PS.reflparamsf) == 80
+
+
+def test_new_entry_points_report_errors_without_a_device():
+    """The kernel-selection hook rejects unknown kinds, and every entry point
+    added for the figures of merit and the FP32 interface reports an error
+    (here: no device) instead of crashing or computing on the CPU."""
+    from pypo_amd import bind_gpu as BG
+    from pypo_amd import efficiencies as E
+    lib = B.loadHIPlib()
+    buf = ctypes.create_string_buffer(256)
+    lib.pypo_hip_last_error(buf, 256)
+    assert lib.pypo_hip_set_ff_kernel(5) != 0
+    assert lib.pypo_hip_last_error(buf, 256) != 0
+    assert lib.pypo_hip_set_ff_kernel(1) == 0 and lib.pypo_hip_set_ff_kernel(0) == 0
+    with pytest.raises(ValueError):
+        B.set_ff_kernel("tensor")
+    with pytest.raises(E.ApertureError):
+        E.check_aperDict({"shape": "ellipse", "outer": [1, 1]})        # no "inner"
+    with pytest.raises(E.ApertureError):
+        E.aper_struct({"shape": "hexagon", "outer": [1, 1], "inner": [0, 0]})
+    if lib.pypo_hip_device_count() > 0:
+        pytest.skip("a device is visible here")
+    E._declare(lib)
+    out = (ctypes.c_double * 8)()
+    d = ctypes.POINTER(ctypes.c_double)()
+    c = PS.reflcontainer()
+    assert lib.pypo_hip_aperture_sums_dev(ctypes.byref(c), d, d, E.aper_struct(None), out, None) != 0
+    assert lib.pypo_hip_power_sums_dev(d, d, d, d, 3, out, None) != 0
+    f = PS.cframe()
+    assert lib.pypo_hip_frame_stats_dev(ctypes.byref(f), out, None) != 0
+    lib.pypo_hip_last_error(buf, 256)
+    # the FP32 entry points return void and record the failure
+    BG.loadGPUlib()
+    src = G.plane({"gmode": "xy", "lims_x": [-1, 1], "lims_y": [-1, 1], "gridsize": [3, 3]})
+    from pypo_amd import types as T
+    z = np.zeros((3, 3), complex)
+    with pytest.raises(B.PyPOHIPError):
+        BG.PyPO_GPUf(src, src, {"mode": "EH", "k": 1.0, "s_current": T.currents(z, z, z, z, z, z)})
+    fr = T.frame(2, np.zeros(2), np.zeros(2), np.zeros(2), np.zeros(2), np.zeros(2), np.ones(2))
+    with pytest.raises(B.PyPOHIPError):
+        BG.RT_GPUf({"fr_in": fr, "t_name": src, "tol": 1e-6, "t0": 1.0})
