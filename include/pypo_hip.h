@@ -338,6 +338,13 @@ int pypo_hip_set_sum_shape(int tpl, int want_wgs);
  * accumulation), PYPO_FF_VECTOR on the vector ALUs (FP32 FMAs).  Both fold
  * per-tile sums into FP64 and meet the same accuracy bar; a tuning and
  * cross-check hook.  Returns 0, or ERR_ARG. */
+/* Fault injection for tests of the error path: the next n device-buffer
+ * requests fail as a failed hipMalloc would (ERR_ALLOC recorded, the call
+ * returns it; nothing is allocated or freed).  The environment variable
+ * PYPO_HIP_FAIL_ALLOCS sets n at initialisation.  0 disables.  No
+ * counterpart in the reference, which has no error channel. */
+int pypo_hip_inject_alloc_failures(int n);
+
 #define PYPO_FF_MATRIX 0
 #define PYPO_FF_VECTOR 1
 int pypo_hip_set_ff_kernel(int kind);

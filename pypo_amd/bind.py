@@ -98,6 +98,8 @@ def loadHIPlib():
         lib.pypo_hip_set_sum_shape.restype = i
         lib.pypo_hip_set_ff_kernel.argtypes = [i]
         lib.pypo_hip_set_ff_kernel.restype = i
+        lib.pypo_hip_inject_alloc_failures.argtypes = [i]
+        lib.pypo_hip_inject_alloc_failures.restype = i
         RC, CB = P(PS.reflcontainer), P(PS.c2Bundle)
         lib.pypo_hip_displace_dev.argtypes = [RC, P(d), RC, ctypes.c_void_p]
         lib.pypo_hip_displace_dev.restype = i

@@ -8,6 +8,9 @@
 #include <random>
 #include <vector>
 
+#include <algorithm>
+#include <cstdlib>
+
 #include "common.hpp"
 
 namespace pypo {
@@ -35,6 +38,9 @@ void *g_pool[SLOT_COUNT] = {};
 size_t g_pool_bytes[SLOT_COUNT] = {};
 hipStream_t g_stream = nullptr;
 hipEvent_t g_done = nullptr;        // completion of the last enqueued call
+hipStream_t g_open = nullptr;       // stream of a call that stopped before end_call
+bool g_open_valid = false;
+int g_fail_allocs = 0;              // fault injection: pool requests left to fail
 int g_precision = PYPO_PREC_MIXED;
 int g_rt_max_iter = 50;
 bool g_timing = false;
@@ -49,17 +55,27 @@ int ensure_init() {
     }
     PYPO_HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     PYPO_HIP_TRY(hipEventCreateWithFlags(&g_done, hipEventDisableTiming));
+    if (const char *v = std::getenv("PYPO_HIP_FAIL_ALLOCS")) g_fail_allocs = std::max(0, std::atoi(v));
     return 0;
 }
 
-// Order this call after every earlier call (they share pooled buffers).
+// Order this call after every earlier call (they share pooled buffers).  A
+// call that stopped on an error after enqueuing work never reached end_call:
+// its stream is marked now, so this call also waits for that work.
 int begin_call(hipStream_t st) {
+    if (g_open_valid) {
+        (void)hipEventRecord(g_done, g_open);
+        g_open_valid = false;
+    }
     if (g_done) PYPO_HIP_TRY(hipStreamWaitEvent(st, g_done, 0));
+    g_open = st;
+    g_open_valid = true;
     return 0;
 }
 
 int end_call(hipStream_t st) {
     PYPO_HIP_TRY(hipEventRecord(g_done, st));
+    g_open_valid = false;
     return 0;
 }
 
@@ -74,6 +90,12 @@ hipStream_t lib_stream(int *err) {
 
 void *pool_get(int slot, size_t bytes, int *err) {
     *err = 0;
+    if (g_fail_allocs > 0) {
+        g_fail_allocs--;
+        set_error(ERR_ALLOC, "injected allocation failure (pypo_hip_inject_alloc_failures)");
+        *err = ERR_ALLOC;
+        return nullptr;
+    }
     if (bytes == 0) bytes = 256;
     if (g_pool_bytes[slot] >= bytes) return g_pool[slot];
     if (g_pool[slot]) {
@@ -814,6 +836,13 @@ int pypo_hip_sweep_destroy(void *handle) {
 int pypo_hip_set_sum_shape(int tpl, int want_wgs) {
     std::lock_guard<std::recursive_mutex> lock(g_mu);
     return set_sum_shape(tpl, want_wgs);
+}
+
+int pypo_hip_inject_alloc_failures(int n) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    PYPO_REQUIRE(n >= 0, ERR_ARG, "inject_alloc_failures: n must be >= 0");
+    g_fail_allocs = n;
+    return 0;
 }
 
 int pypo_hip_set_ff_kernel(int kind) {
