@@ -597,7 +597,7 @@ void trace(const reflparams &tp, cframe *in, cframe *out, int nthreads, double t
                 t1 = newton_step(tp.type, t, x, y, z, dx, dy, dz, a, b, c);
                 chk = std::fabs(t1 - t);
                 t = t1;
-                it++;
+                if (max_iter > 0) it++;   // uncapped: no counter, so none to overflow
             }
             bool dead = std::fabs(std::round(dx)) == 0 && std::fabs(std::round(dy)) == 0 &&
                         std::fabs(std::round(dz)) == 0;
