@@ -7,8 +7,8 @@
 //   S_q(r) = sum_s W(s, r) C_q(s),   W = exp(-j t k r_s.r),  C = w A (J, M),
 // a (targets x sources) by (sources x 12) complex matrix product whose left
 // factor is generated on the fly.  Each lane computes W for four sources and
-// one target -- FP64 phase in revolutions, v_fract_f64, FP32 v_sin/v_cos --
-// and the products run as v_mfma_f32_16x16x32_f16 with FP32 accumulation:
+// one target -- phase in revolutions, FP32 v_sin/v_cos -- and the products
+// run as v_mfma_f32_16x16x32_f16 with FP32 accumulation:
 //
 //   A (16 targets x 32) = [cos, sin] of 16 sources, interleaved along K;
 //   B (32 x 16 columns)  = per source s and output column q = 2a + part
@@ -24,6 +24,23 @@
 // sums are multiplied back and folded into FP64 after every tile, like the
 // vector kernel's.  The operand split costs three VALU instructions per pair
 // against the twelve FP32 FMAs the vector kernel spends on the products.
+//
+// Phase.  The four sources a lane takes per K-step are consecutive records,
+// i.e. neighbours on the source grid.  The second of them (the leader) gets
+// the full FP64 phase rev = r_s . q (q = t k r / 2 pi, FP64), v_fract_f64.
+// The other three (followers) add their FP32 offset from the leader,
+// d_s = (r_s - r_leader) t k / 2 pi, dotted with the FP32 unit direction:
+//   f_s = fract(f_leader + d_s . u)          (three FP32 FMAs, v_fract_f32)
+// instead of three FP64 FMAs, v_fract_f64 and a conversion each.  The FP32
+// rounding of that sum is at most a few ulps of its partial sums, which are
+// bounded per workgroup by 1 + sum_i |d_i| (|a_i| + sin(theta)) (a, theta:
+// the cone of the workgroup's target directions).  A K-step takes the
+// follower form only when that bound is <= kLocalRev for all its followers
+// (a wave-uniform flag per K-step, set when the tile is staged); otherwise
+// -- across a grid-row wrap, past the end of the sources, for wide target
+// cones on coarse grids -- all four sources get the FP64 phase.  The error
+// that stays is that of the FP32 direction u (2^-24 relative) times |d|,
+// a few 1e-7 revolutions for neighbours a few wavelengths apart.
 //
 // Workgroup: 4 waves x MB M-blocks of 16 targets.  Per tile each lane takes
 // one FP32 source record from HBM (prefetched during the previous tile),
@@ -45,6 +62,10 @@ typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int KSTEPS = TILE / 16;   // 16 sources (32 K rows: cos and sin) per MFMA
+
+// Largest bound on the follower phase sums (revolutions): FP32 rounds such a
+// sum to 2^-24 * 8 = 5e-7 revolutions at worst.
+constexpr float kLocalRev = 8.f;
 
 // 1 / (power of two at or above cmax); 1 for an all-zero tile.
 __device__ __forceinline__ float tile_inv_scale(float cmax) {
@@ -95,67 +116,101 @@ __device__ __forceinline__ void build_b(const SrcRec32 &r, int sl, float inv, ha
 }
 
 // Source coordinates of a tile in LDS, one array per coordinate so that the
-// four lane groups of a wave (4 consecutive sources each) read distinct banks.
+// four lane groups of a wave (4 consecutive sources each) read distinct banks;
+// d* are the follower offsets from their leader in revolutions (FP32).
 struct TileXYZ {
     double x[TILE], y[TILE], z[TILE];
     float nx[TILE], ny[TILE], nz[TILE];
+    float dx[TILE], dy[TILE], dz[TILE];
+    int fast[4];   // per wave: 1 when all its followers may take the follower form
 };
 
-// One 256-source tile: 16 K-steps, each 3 MFMAs per M-block.  The MFMAs of
-// M-block m are issued between the pair computations of M-block m + 1
-// (sched_barrier pins the order): the matrix core runs while the wave
-// keeps issuing vector work.  Left to itself the compiler clusters the
-// three dependent MFMAs and the SIMD runs them and the vector work in turn.
-template <int MB, bool MASK>
+// Slot of the leader of source slot `sl`'s group of four.
+__device__ __forceinline__ int leader_slot(int sl) { return (sl & ~3) | 1; }
+
+// One K-step (16 sources) for the MB M-blocks: the pair computations and
+// 3 MFMAs per M-block.  The MFMAs of M-block m are issued between the pair
+// computations of M-block m + 1 (sched_barrier pins the order): the matrix
+// core runs while the wave keeps issuing vector work.  Left to itself the
+// compiler clusters the three dependent MFMAs and the SIMD runs them and the
+// vector work in turn.  FAST: the follower form of the phase (see above).
+template <int MB, bool MASK, bool FAST>
+__device__ __forceinline__ void ff_kstep(const TileXYZ &tx, const half8 &b_hi, const half8 &b_lo, int s0,
+                                         const double (&qx)[MB], const double (&qy)[MB], const double (&qz)[MB],
+                                         const float4 *tq, int lt0, floatx4 (&acc)[MB]) {
+    double sx[4], sy[4], sz[4];
+    float dx[4], dy[4], dz[4], nx[4], ny[4], nz[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (!FAST || j == 1) { sx[j] = tx.x[s0 + j]; sy[j] = tx.y[s0 + j]; sz[j] = tx.z[s0 + j]; }
+        if (FAST && j != 1) { dx[j] = tx.dx[s0 + j]; dy[j] = tx.dy[s0 + j]; dz[j] = tx.dz[s0 + j]; }
+        if (MASK) { nx[j] = tx.nx[s0 + j]; ny[j] = tx.ny[s0 + j]; nz[j] = tx.nz[s0 + j]; }
+    }
+    unsigned ah[2][4], al[2][4];
+    float f_lead = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+    auto load_u = [&](int m) {
+        const float4 &t = tq[lt0 + 16 * m];
+        ux = t.x; uy = t.y; uz = t.z;
+    };
+    // j-th computation of an M-block: the leader (source 1) first
+    auto src_of = [](int j) { return FAST ? (j == 0 ? 1 : (j == 1 ? 0 : j)) : j; };
+    auto pair = [&](int m, int j, unsigned &h, unsigned &l) {
+        float f;
+        if (!FAST || j == 1) {
+            const double rev = __builtin_fma(sz[j], qz[m], __builtin_fma(sy[j], qy[m], sx[j] * qx[m]));
+            f = (float)__builtin_amdgcn_fract(rev);
+            if (FAST) f_lead = f;
+        } else {
+            f = __builtin_amdgcn_fractf(
+                __builtin_fmaf(dz[j], uz, __builtin_fmaf(dy[j], uy, __builtin_fmaf(dx[j], ux, f_lead))));
+        }
+        float sn, cs;
+        sincos_rev_f32(f, sn, cs);
+        if (MASK) {
+            const float d = __builtin_fmaf(nz[j], uz, __builtin_fmaf(ny[j], uy, nx[j] * ux));
+            cs = (d < 0.f) ? 0.f : cs;
+            sn = (d < 0.f) ? 0.f : sn;
+        }
+        split_pair_fast(cs, sn, h, l);
+    };
+    if (FAST || MASK) load_u(0);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int js = src_of(j);
+        pair(0, js, ah[0][js], al[0][js]);
+    }
+#pragma unroll
+    for (int m = 0; m < MB; m++) {
+        const int c = m & 1, n = c ^ 1;
+        const half8 a_hi = __builtin_bit_cast(half8, ah[c]), a_lo = __builtin_bit_cast(half8, al[c]);
+        if ((FAST || MASK) && m + 1 < MB) load_u(m + 1);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (j < 3) {
+                __builtin_amdgcn_sched_barrier(0);
+                if (j == 0) acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_hi, acc[m], 0, 0, 0);
+                if (j == 1) acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_lo, acc[m], 0, 0, 0);
+                if (j == 2) acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_lo, b_hi, acc[m], 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            const int js = src_of(j);
+            if (m + 1 < MB) pair(m + 1, js, ah[n][js], al[n][js]);
+        }
+    }
+}
+
+// One 256-source tile: 16 K-steps, all in the follower form or none (one
+// loop body per form keeps the accumulators in place).
+template <int MB, bool MASK, bool FAST>
 __device__ __forceinline__ void ff_tile_mfma(const TileXYZ &tx, const half8 (*bhi)[64], const half8 (*blo)[64],
                                              int lane, const double (&qx)[MB], const double (&qy)[MB],
-                                             const double (&qz)[MB], const float (*tu)[64 * MB], int lt0,
+                                             const double (&qz)[MB], const float4 *tq, int lt0,
                                              floatx4 (&acc)[MB]) {
     const int kb = lane >> 4;
 #pragma unroll 1
     for (int ks = 0; ks < KSTEPS; ks++) {
         const half8 b_hi = bhi[ks][lane], b_lo = blo[ks][lane];
-        const int s0 = ks * 16 + kb * 4;
-        double sx[4], sy[4], sz[4];
-        float nx[4], ny[4], nz[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            sx[j] = tx.x[s0 + j]; sy[j] = tx.y[s0 + j]; sz[j] = tx.z[s0 + j];
-            if (MASK) { nx[j] = tx.nx[s0 + j]; ny[j] = tx.ny[s0 + j]; nz[j] = tx.nz[s0 + j]; }
-        }
-        unsigned ah[2][4], al[2][4];
-        auto pair = [&](int m, int j, unsigned &h, unsigned &l) {
-            const double rev = __builtin_fma(sz[j], qz[m], __builtin_fma(sy[j], qy[m], sx[j] * qx[m]));
-            const float f = (float)__builtin_amdgcn_fract(rev);
-            float sn, cs;
-            sincos_rev_f32(f, sn, cs);
-            if (MASK) {
-                // target direction from LDS (kept out of registers: only masked tiles need it)
-                const int lt = lt0 + 16 * m;
-                const float d = __builtin_fmaf(nz[j], tu[2][lt], __builtin_fmaf(ny[j], tu[1][lt], nx[j] * tu[0][lt]));
-                cs = (d < 0.f) ? 0.f : cs;
-                sn = (d < 0.f) ? 0.f : sn;
-            }
-            split_pair_fast(cs, sn, h, l);
-        };
-#pragma unroll
-        for (int j = 0; j < 4; j++) pair(0, j, ah[0][j], al[0][j]);
-#pragma unroll
-        for (int m = 0; m < MB; m++) {
-            const int c = m & 1, n = c ^ 1;
-            const half8 a_hi = __builtin_bit_cast(half8, ah[c]), a_lo = __builtin_bit_cast(half8, al[c]);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if (j < 3) {
-                    __builtin_amdgcn_sched_barrier(0);
-                    if (j == 0) acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_hi, acc[m], 0, 0, 0);
-                    if (j == 1) acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_lo, acc[m], 0, 0, 0);
-                    if (j == 2) acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_lo, b_hi, acc[m], 0, 0, 0);
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-                if (m + 1 < MB) pair(m + 1, j, ah[n][j], al[n][j]);
-            }
-        }
+        ff_kstep<MB, MASK, FAST>(tx, b_hi, b_lo, ks * 16 + kb * 4, qx, qy, qz, tq, lt0, acc);
     }
 }
 
@@ -169,7 +224,7 @@ __device__ __forceinline__ void ff_mfma_body(const SrcRec32 *__restrict__ src, c
                                              double *__restrict__ out) {
     __shared__ TileXYZ tx;
     __shared__ half8 bhi[KSTEPS][64], blo[KSTEPS][64];
-    __shared__ float tu[3][64 * MB];   // unit directions of the workgroup's targets
+    __shared__ float4 tq[64 * MB];     // unit directions of the workgroup's targets (x, y, z, 0)
     __shared__ float scratch[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int row = lane & 15, kb = lane >> 4;
@@ -200,7 +255,7 @@ __device__ __forceinline__ void ff_mfma_body(const SrcRec32 *__restrict__ src, c
             }
             fx[m] = (float)x; fy[m] = (float)y; fz[m] = (float)z;
             const int lt = wave * 16 * MB + 16 * m + row;
-            tu[0][lt] = fx[m]; tu[1][lt] = fy[m]; tu[2][lt] = fz[m];
+            tq[lt] = make_float4(fx[m], fy[m], fz[m], 0.f);
             qx[m] = x * rev_scale; qy[m] = y * rev_scale; qz[m] = z * rev_scale;
             if (m == 0) any = t < n_tgt;
         }
@@ -231,17 +286,40 @@ __device__ __forceinline__ void ff_mfma_body(const SrcRec32 *__restrict__ src, c
         const float inv = tile_inv_scale(tm.cmax);
         {
             const SrcRec32 r = __builtin_bit_cast(SrcRec32, ld.buf);   // this lane's record
+            // offset from the group's leader (same wave), in revolutions
+            const int ls = leader_slot(lane);
+            const float ddx = (float)((r.x - __shfl(r.x, ls, 64)) * rev_scale);
+            const float ddy = (float)((r.y - __shfl(r.y, ls, 64)) * rev_scale);
+            const float ddz = (float)((r.z - __shfl(r.z, ls, 64)) * rev_scale);
+            // per-component bound on |u| over the workgroup's targets: |a_i| + sin(theta)
+            float ubx = 1.f, uby = 1.f, ubz = 1.f;
+            if (cone.cos_half > 0.f) {
+                const float sn = sqrtf(fmaxf(0.f, 1.f - cone.cos_half * cone.cos_half)) + 1e-3f;
+                ubx = fminf(1.f, fabsf(cone.ax) + sn);
+                uby = fminf(1.f, fabsf(cone.ay) + sn);
+                ubz = fminf(1.f, fabsf(cone.az) + sn);
+            }
+            const float bound = 1.f + fabsf(ddx) * ubx + fabsf(ddy) * uby + fabsf(ddz) * ubz;
+            const bool ok = (lane == ls) || (bound <= kLocalRev);   // NaN: not ok
+            const unsigned long long all_ok = __ballot(ok);
             __syncthreads();
             tx.x[tid] = r.x; tx.y[tid] = r.y; tx.z[tid] = r.z;
             tx.nx[tid] = r.nx; tx.ny[tid] = r.ny; tx.nz[tid] = r.nz;
+            tx.dx[tid] = ddx; tx.dy[tid] = ddy; tx.dz[tid] = ddz;
+            if (lane == 0) tx.fast[wave] = all_ok == ~0ull;
             build_b(r, tid, inv, bhi, blo);
             __syncthreads();
         }
         if (base + TILE < s_end) ld.load(g, base + TILE + tid, s_end);
-        if (cones_may_shadow(tm.ax, tm.ay, tm.az, tm.cos_half, cone))
-            ff_tile_mfma<MB, true>(tx, bhi, blo, lane, qx, qy, qz, tu, wave * 16 * MB + row, acc);
-        else
-            ff_tile_mfma<MB, false>(tx, bhi, blo, lane, qx, qy, qz, tu, wave * 16 * MB + row, acc);
+        const bool fast = __builtin_amdgcn_readfirstlane(tx.fast[0] & tx.fast[1] & tx.fast[2] & tx.fast[3]);
+        const int lt0 = wave * 16 * MB + row;
+        if (cones_may_shadow(tm.ax, tm.ay, tm.az, tm.cos_half, cone)) {
+            if (fast) ff_tile_mfma<MB, true, true>(tx, bhi, blo, lane, qx, qy, qz, tq, lt0, acc);
+            else ff_tile_mfma<MB, true, false>(tx, bhi, blo, lane, qx, qy, qz, tq, lt0, acc);
+        } else {
+            if (fast) ff_tile_mfma<MB, false, true>(tx, bhi, blo, lane, qx, qy, qz, tq, lt0, acc);
+            else ff_tile_mfma<MB, false, false>(tx, bhi, blo, lane, qx, qy, qz, tq, lt0, acc);
+        }
         const double scale = 1.0 / (double)inv;   // a power of two: exact
 #pragma unroll
         for (int m = 0; m < MB; m++) {

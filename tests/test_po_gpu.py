@@ -104,6 +104,70 @@ def test_ff_matrix_kernel_edge_tiles(hip, port):
     assert err <= 1e-5
 
 
+def _follower_tiles(src_grid, th, ph, k):
+    """Which 256-source tiles the matrix kernel runs in its follower-phase form
+    (one FP64 phase per group of four consecutive sources, FP32 offsets for
+    the other three): all follower sums bounded by 8 revolutions, the bound
+    taken over the cone of the target directions (one workgroup here)."""
+    u = np.stack([np.cos(ph) * np.sin(th), np.sin(ph) * np.sin(th), np.cos(th)])
+    a = u.sum(axis=1)
+    a /= np.linalg.norm(a)
+    cos_half = np.min(a @ u)
+    ub = np.minimum(1.0, np.abs(a) + np.sqrt(max(0.0, 1 - cos_half ** 2)) + 1e-3) if cos_half > 0 else np.ones(3)
+    r = np.stack([src_grid.x, src_grid.y, src_grid.z]).reshape(3, -1)
+    n = r.shape[1]
+    flags = []
+    for t0 in range(0, n, 256):
+        ok = True
+        for g0 in range(t0, min(n, t0 + 256), 4):
+            lead = g0 + 1
+            for s in range(g0, g0 + 4):
+                if s == lead:
+                    continue
+                if s >= n or lead >= n:
+                    ok = False            # padding record far away
+                    continue
+                d = np.abs((r[:, s] - r[:, lead]) * k / (2 * np.pi))
+                ok &= 1 + d @ ub <= 8.0
+        flags.append(ok)
+    return np.array(flags)
+
+
+@pytest.mark.parametrize("case", ["follower", "fp64", "both"])
+def test_ff_matrix_phase_forms(hip, port, case):
+    """The matrix kernel's two phase forms against the CPU checker and the
+    vector kernel: a 64 x 64 plane at 0.5 mm spacing, lambda = 2 mm, where
+    every tile takes the follower form; the same plane at lambda = 0.05 mm
+    (neighbours 10 revolutions apart: every tile takes the full FP64 phase);
+    and 16 x 301 points at lambda = 2 mm, where the groups of four that
+    straddle a row end (jumping across the plane) put the tiles holding them
+    on the FP64 form and the others on the follower form."""
+    lam, nx, ny = {"follower": (2.0, 64, 64), "fp64": (0.05, 64, 64), "both": (2.0, 16, 301)}[case]
+    src = G.plane({"gmode": "xy", "lims_x": [-16, 16], "lims_y": [-16, 16], "gridsize": [nx, ny]})
+    ff = G.plane({"gmode": "AoE", "lims_Az": [-30, 30], "lims_El": [-30, 30], "gridsize": [23, 19]})
+    k = 2 * np.pi / lam
+    rng = np.random.default_rng(11)
+    cur = T.currents(*[rng.normal(size=(nx, ny)) + 1j * rng.normal(size=(nx, ny)) for _ in range(6)])
+    gs = B.generate_grid(src)
+    gt = B.generate_grid(ff, transform=False)
+    flags = _follower_tiles(gs, np.ravel(gt.x), np.ravel(gt.y), k)
+    expect = {"follower": flags.all(), "fp64": not flags.any(), "both": flags.any() and not flags.all()}
+    assert expect[case], flags
+    d = {"mode": "FF", "k": k, "s_current": cur, "precision": "mixed"}
+    ref, _, _ = OB.propagate(port, "FF", src, ff, cur, k)
+    outs = {}
+    try:
+        for kind in ("vector", "matrix"):
+            B.set_ff_kernel(kind)
+            outs[kind] = flat(B.PyPO_HIPd(src, ff, d))
+            err = group_err(outs[kind], flat(ref))
+            print(f"{case} ({flags.sum()}/{flags.size} follower tiles) {kind}: {err:.2e}")
+            assert err <= TOL["mixed"]
+    finally:
+        B.set_ff_kernel("matrix")
+    assert group_err(outs["matrix"], outs["vector"]) <= 2e-6
+
+
 def _cfg1():
     """Plain PO configuration: vector Gaussian-like feed, lambda = 3 mm, on a
     101 x 101 plane over +-20 mm at the focus of a paraboloid (f = 0.5 m,
