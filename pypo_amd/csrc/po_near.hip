@@ -58,10 +58,11 @@ __global__ __launch_bounds__(256) void k_near_mixed(const SrcRec32 *__restrict__
         ld.store(reinterpret_cast<uint4 *>(tile), tid);
         __syncthreads();
         if (base + TILE < s_end) ld.load(g, base + TILE + tid, s_end);
+        const int n = min(TILE, s_end - base);   // padding records add exactly zero: skip them
         if (meta[base / TILE].has_m)
-            near_tile_mixed<TPL, true, WANT_E>(tile, px, py, pz, rev_scale, kinv, tf, acc);
+            near_tile_mixed<TPL, true, WANT_E>(tile, px, py, pz, rev_scale, kinv, tf, acc, n);
         else
-            near_tile_mixed<TPL, false, WANT_E>(tile, px, py, pz, rev_scale, kinv, tf, acc);
+            near_tile_mixed<TPL, false, WANT_E>(tile, px, py, pz, rev_scale, kinv, tf, acc, n);
 #pragma unroll
         for (int j = 0; j < TPL; j++)
 #pragma unroll
@@ -82,9 +83,9 @@ template <int TPL, bool HAS_M>
 __device__ __forceinline__ void near_tile_fp64(const SrcRec64 *tile, const double (&px)[TPL],
                                                const double (&py)[TPL], const double (&pz)[TPL],
                                                double rev_scale, double k, double tdir,
-                                               double (&acc)[TPL][12]) {
+                                               double (&acc)[TPL][12], int n) {
 #pragma unroll 1
-    for (int s = 0; s < TILE; s++) {
+    for (int s = 0; s < n; s++) {
         const SrcRec64 &r = tile[s];
         const double *cj = r.c, *cm = r.c + 6;
 #pragma unroll
@@ -188,10 +189,11 @@ __global__ __launch_bounds__(256) void k_near_fp64(const SrcRec64 *__restrict__ 
         ld.store(reinterpret_cast<uint4 *>(tile), tid);
         __syncthreads();
         if (base + TILE < s_end) ld.load(g, base + TILE + tid, s_end);
+        const int n = min(TILE, s_end - base);
         if (meta[base / TILE].has_m)
-            near_tile_fp64<TPL, true>(tile, px, py, pz, rev_scale, k, tdir, acc);
+            near_tile_fp64<TPL, true>(tile, px, py, pz, rev_scale, k, tdir, acc, n);
         else
-            near_tile_fp64<TPL, false>(tile, px, py, pz, rev_scale, k, tdir, acc);
+            near_tile_fp64<TPL, false>(tile, px, py, pz, rev_scale, k, tdir, acc, n);
     }
 #pragma unroll
     for (int j = 0; j < TPL; j++) {

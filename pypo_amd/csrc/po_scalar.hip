@@ -38,8 +38,9 @@ __global__ __launch_bounds__(256) void k_scalar_mixed(const SrcRecS32 *__restric
         ld.store(reinterpret_cast<uint4 *>(tile), tid);
         __syncthreads();
         if (base + TILE < s_end) ld.load(g, base + TILE + tid, s_end);
+        const int n = min(TILE, s_end - base);   // padding records add exactly zero: skip them
 #pragma unroll 2
-        for (int s = 0; s < TILE; s++) {
+        for (int s = 0; s < n; s++) {
             const SrcRecS32 &r = tile[s];
 #pragma unroll
             for (int j = 0; j < TPL; j++) {
@@ -103,8 +104,9 @@ __global__ __launch_bounds__(256) void k_scalar_fp64(const SrcRecS64 *__restrict
         ld.store(reinterpret_cast<uint4 *>(tile), tid);
         __syncthreads();
         if (base + TILE < s_end) ld.load(g, base + TILE + tid, s_end);
+        const int n = min(TILE, s_end - base);
 #pragma unroll 1
-        for (int s = 0; s < TILE; s++) {
+        for (int s = 0; s < n; s++) {
             const SrcRecS64 &r = tile[s];
 #pragma unroll
             for (int j = 0; j < TPL; j++) {

@@ -68,9 +68,9 @@ template <int TPL, bool HAS_M, bool WANT_E>
 __device__ __forceinline__ void near_tile_mixed(const SrcRec32 *tile, const double (&px)[TPL],
                                                 const double (&py)[TPL], const double (&pz)[TPL],
                                                 double rev_scale, float kinv, float tf,
-                                                float (&acc)[TPL][12]) {
+                                                float (&acc)[TPL][12], int n = TILE) {
 #pragma unroll 1
-    for (int s = 0; s < TILE; s++) {
+    for (int s = 0; s < n; s++) {   // n: the tile's records before the padding
         const SrcRec32 &r = tile[s];
         float cj[6], cm[6];
 #pragma unroll

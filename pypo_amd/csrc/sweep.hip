@@ -177,10 +177,11 @@ __global__ __launch_bounds__(256) void k_sweep_near(const StageInst *__restrict_
             __syncthreads();
             if (base + TILE < item.s1) ld.load(g, base + TILE + tid, item.s1);
             // JM needs H only: rows 6..11 of the partials
+            const int n = min(TILE, item.s1 - base);
             if (in.meta[base / TILE].has_m)
-                near_tile_mixed<TPL, true, false>(tile, px, py, pz, in.rev_scale, kinv, tf, acc);
+                near_tile_mixed<TPL, true, false>(tile, px, py, pz, in.rev_scale, kinv, tf, acc, n);
             else
-                near_tile_mixed<TPL, false, false>(tile, px, py, pz, in.rev_scale, kinv, tf, acc);
+                near_tile_mixed<TPL, false, false>(tile, px, py, pz, in.rev_scale, kinv, tf, acc, n);
 #pragma unroll
             for (int j = 0; j < TPL; j++)
 #pragma unroll
