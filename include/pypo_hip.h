@@ -184,8 +184,10 @@ enum pypo_hip_precision {
     PYPO_PREC_FP64 = 1     /* everything in FP64 */
 };
 
-/* Last error of the calling thread: returns 0 if none (and clears it),
- * otherwise a non-zero code and the message copied into buf (n bytes). */
+/* Error of the calling thread: the first failure since the last read (later
+ * failures do not overwrite it; a successful call does not clear it).
+ * Returns 0 if none, otherwise a non-zero code and the message copied into
+ * buf (n bytes); either way the slot is cleared. */
 int pypo_hip_last_error(char *buf, int n);
 
 /* Library version string. */

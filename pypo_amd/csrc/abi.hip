@@ -28,8 +28,6 @@ void set_error(int code, const std::string &msg) {
     }
 }
 
-bool has_error() { return t_err_code != 0; }
-
 // ---- library state ------------------------------------------------------------
 namespace {
 

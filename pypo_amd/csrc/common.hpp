@@ -16,7 +16,6 @@ namespace pypo {
 // Every failure is recorded per calling thread and surfaced by
 // pypo_hip_last_error(); nothing calls exit() or resets the device.
 void set_error(int code, const std::string &msg);
-bool has_error();
 
 #define PYPO_HIP_TRY(expr)                                                        \
     do {                                                                          \
