@@ -101,3 +101,39 @@ def test_bad_sweep_arguments_raise(hip):
         SW.Sweep(feeds[:2], dishes[:2], [1.0, -1.0], curs[:2], ff)
     with pytest.raises(B.PyPOHIPError):
         SW.Sweep(feeds[:1], [dict(dishes[0], rms=0.1)], ks[:1], curs[:1], ff)
+
+
+def test_full_size_sweep(hip, port):
+    """The sweep at full size: 64 frequencies 80-120 GHz x feed offsets
+    (0, +2, -2, +4) mm = 256 instances, dish uv grids 201^2 .. 401^2, 25^2
+    feeds, one 181^2 AoE far field over +-1 deg, one hipGraph replay.  Three
+    instances across the size range against single device propagations (JM
+    to 1e-9, FF to 1e-6 of peak), and the largest instance (120 GHz, +4 mm,
+    401^2) against the CPU checker within the mixed bar: JM on every dish
+    point, the far field on every 97th target plus the beam centre."""
+    from pypo_amd import device as D
+    feeds, dishes, ks, curs, ff, meta = SW.frequency_offset_sweep()
+    s = SW.Sweep(feeds, dishes, ks, curs, ff, tpl=2, chunk_tiles=4096, wgs_per_cu=4, use_graph=True)
+    try:
+        assert s.n_inst == 256 and meta[-1]["n_dish"] == 401 and meta[0]["n_dish"] == 201
+        s.run()
+        g_ff = D.generate_grid(ff)
+        for i in (0, 130, 255):
+            g_feed, g_dish = D.generate_grid(feeds[i]), D.generate_grid(dishes[i])
+            jm = D.propagate("JM", g_feed, g_dish, D.currents_tensor(curs[i]), feeds[i], ks[i])
+            e = D.propagate("FF", g_dish, g_ff, jm, dishes[i], ks[i])
+            e_jm = group_err(flat(s.read_currents(i)), D.tensor_to_complex6(jm))
+            e_ff = group_err(flat(s.read(i)), D.tensor_to_complex6(e))
+            print(f"instance {i} {meta[i]}: vs single JM {e_jm:.2e} FF {e_ff:.2e}")
+            assert e_jm < 1e-9 and e_ff < 1e-6
+        i = 255
+        jm_ref, _, _ = OB.propagate(port, "JM", feeds[i], dishes[i], curs[i], ks[i])
+        e_jm = group_err(flat(s.read_currents(i)), flat(jm_ref))
+        n_ff = 181 * 181
+        idx = np.union1d(np.arange(0, n_ff, 97), [90 * 181 + 90])
+        ff_ref, _ = OB.farfield_subset(port, dishes[i], ff, jm_ref, ks[i], idx)
+        e_ff = group_err(flat(s.read(i))[:, idx], ff_ref)
+        print(f"instance {i} vs CPU checker: JM {e_jm:.2e}, FF ({idx.size} targets) {e_ff:.2e}")
+        assert e_jm <= 1e-5 and e_ff <= 1e-5
+    finally:
+        s.close()
