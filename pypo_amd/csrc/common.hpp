@@ -119,9 +119,10 @@ static_assert(sizeof(SrcRecS64) == 48, "SrcRecS64 must be 48 bytes");
 
 // ---- device math -----------------------------------------------------------
 
-// sin and cos of 2*pi*f for a reduced argument f in [-0.5, 0.5] revolutions,
-// using the hardware transcendental unit (v_sin_f32 / v_cos_f32 take their
-// argument in revolutions).
+// sin and cos of 2*pi*f, f in revolutions, on the hardware transcendental unit
+// (v_sin_f32 / v_cos_f32 take their argument in revolutions).  Callers pass a
+// reduced f in [-0.5, 1), except the far field's follower phases, |f| <= 9:
+// the unit reduces those itself to within one ulp of its values on fract(f).
 __device__ __forceinline__ void sincos_rev_f32(float f, float &s, float &c) {
     s = __builtin_amdgcn_sinf(f);
     c = __builtin_amdgcn_cosf(f);

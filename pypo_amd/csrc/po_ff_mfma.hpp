@@ -30,8 +30,11 @@
 // the full FP64 phase rev = r_s . q (q = t k r / 2 pi, FP64), v_fract_f64.
 // The other three (followers) add their FP32 offset from the leader,
 // d_s = (r_s - r_leader) t k / 2 pi, dotted with the FP32 unit direction:
-//   f_s = fract(f_leader + d_s . u)          (three FP32 FMAs, v_fract_f32)
-// instead of three FP64 FMAs, v_fract_f64 and a conversion each.  The FP32
+//   f_s = f_leader + d_s . u                  (three FP32 FMAs)
+// instead of three FP64 FMAs, v_fract_f64 and a conversion each; v_sin_f32
+// and v_cos_f32 reduce such an argument (|f_s| <= 9 revolutions) themselves,
+// to within one ulp (6e-8) of their values on fract(f_s) (measured on gfx950
+// over 2^24 points in [-9, 9]), so no v_fract_f32 is spent.  The FP32
 // rounding of that sum is at most a few ulps of its partial sums, which are
 // bounded per workgroup by 1 + sum_i |d_i| (|a_i| + sin(theta)) (a, theta:
 // the cone of the workgroup's target directions).  A K-step takes the
@@ -82,15 +85,18 @@ __device__ __forceinline__ void split_pair(float c, float s, half2v &hi, half2v 
 }
 
 // The same split for the per-pair A operand: one packed RNE conversion for
-// hi, then lo = c - hi and s - hi rounded to FP16 by v_fma_mix{lo,hi}_f16
-// (FP32 c, s; hi read as FP16 halves; the FMA with -1 is exact before the
-// final rounding).  Measured on gfx950, conversions and the mixed forms issue
-// at about a quarter of the FP32 rate; this three-instruction form beat a
-// mask-and-subtract split with two packed conversions (114.4 vs 111.4 ms).
+// hi; c - hi and s - hi by v_fma_mix_f32 (hi read as FP16 halves, c and s
+// FP32: the differences are exact in FP32); one packed RNE conversion for lo.
+// Issue cost per wave-instruction measured on gfx950 (4 waves per SIMD):
+// v_cvt_pk_f16_f32 5.4 cycles, v_fma_mix_f32 5.5, but v_fma_mix{lo,hi}_f16
+// 9.9 -- so this four-instruction form (21.8) beats cvt_pk + mixlo + mixhi
+// (25.2), with bit-identical lo (checked on 2^24 inputs).
 __device__ __forceinline__ void split_pair_fast(float c, float s, unsigned &hi, unsigned &lo) {
     hi = __builtin_bit_cast(unsigned, half2v{(_Float16)c, (_Float16)s});
-    asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(lo) : "v"(hi), "v"(c));
-    asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(lo) : "v"(hi), "v"(s));
+    float dc, ds;
+    asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(dc) : "v"(hi), "v"(c));
+    asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(ds) : "v"(hi), "v"(s));
+    lo = __builtin_bit_cast(unsigned, half2v{(_Float16)dc, (_Float16)ds});
 }
 
 // This lane's source record (tile slot sl) -> its B entries of every column.
@@ -161,8 +167,8 @@ __device__ __forceinline__ void ff_kstep(const TileXYZ &tx, const half8 &b_hi, c
             f = (float)__builtin_amdgcn_fract(rev);
             if (FAST) f_lead = f;
         } else {
-            f = __builtin_amdgcn_fractf(
-                __builtin_fmaf(dz[j], uz, __builtin_fmaf(dy[j], uy, __builtin_fmaf(dx[j], ux, f_lead))));
+            // |f| <= 1 + kLocalRev: in the hardware's own reduction range (see above)
+            f = __builtin_fmaf(dz[j], uz, __builtin_fmaf(dy[j], uy, __builtin_fmaf(dx[j], ux, f_lead)));
         }
         float sn, cs;
         sincos_rev_f32(f, sn, cs);
