@@ -133,16 +133,19 @@ def _follower_tiles(src_grid, th, ph, k):
     return np.array(flags)
 
 
-@pytest.mark.parametrize("case", ["follower", "fp64", "both"])
+@pytest.mark.parametrize("case", ["follower", "follower_wide", "fp64", "both"])
 def test_ff_matrix_phase_forms(hip, port, case):
     """The matrix kernel's two phase forms against the CPU checker and the
     vector kernel: a 64 x 64 plane at 0.5 mm spacing, lambda = 2 mm, where
-    every tile takes the follower form; the same plane at lambda = 0.05 mm
+    every tile takes the follower form; at lambda = 0.1 mm, still all follower
+    tiles but with offsets d.u up to 5 revolutions (phases up to ~6 handed to
+    v_sin/v_cos unreduced); the same plane at lambda = 0.05 mm
     (neighbours 10 revolutions apart: every tile takes the full FP64 phase);
     and 16 x 301 points at lambda = 2 mm, where the groups of four that
     straddle a row end (jumping across the plane) put the tiles holding them
     on the FP64 form and the others on the follower form."""
-    lam, nx, ny = {"follower": (2.0, 64, 64), "fp64": (0.05, 64, 64), "both": (2.0, 16, 301)}[case]
+    lam, nx, ny = {"follower": (2.0, 64, 64), "follower_wide": (0.1, 64, 64), "fp64": (0.05, 64, 64),
+                   "both": (2.0, 16, 301)}[case]
     src = G.plane({"gmode": "xy", "lims_x": [-16, 16], "lims_y": [-16, 16], "gridsize": [nx, ny]})
     ff = G.plane({"gmode": "AoE", "lims_Az": [-30, 30], "lims_El": [-30, 30], "gridsize": [23, 19]})
     k = 2 * np.pi / lam
@@ -151,7 +154,8 @@ def test_ff_matrix_phase_forms(hip, port, case):
     gs = B.generate_grid(src)
     gt = B.generate_grid(ff, transform=False)
     flags = _follower_tiles(gs, np.ravel(gt.x), np.ravel(gt.y), k)
-    expect = {"follower": flags.all(), "fp64": not flags.any(), "both": flags.any() and not flags.all()}
+    expect = {"follower": flags.all(), "follower_wide": flags.all(), "fp64": not flags.any(),
+              "both": flags.any() and not flags.all()}
     assert expect[case], flags
     d = {"mode": "FF", "k": k, "s_current": cur, "precision": "mixed"}
     ref, _, _ = OB.propagate(port, "FF", src, ff, cur, k)
