@@ -104,3 +104,15 @@ def test_grids_match_golden(hip):
             got = np.stack([np.ravel(getattr(g, a)) for a in ("x", "y", "z", "nx", "ny", "nz", "area")])
             np.testing.assert_allclose(got, gold[f"{name}/{int(tr)}{int(sp)}"], rtol=1e-13, atol=1e-12,
                                        err_msg=name)
+
+
+def test_empty_frame_is_a_no_op(hip):
+    """A frame of no rays traces to a frame of no rays, as the reference's
+    loops simply do not run (RayTrace.h:283-298); the library stays usable."""
+    pla = G.plane({"gmode": "xy", "lims_x": [-1, 1], "lims_y": [-1, 1], "gridsize": [3, 3]})
+    e = np.zeros(0)
+    out = B.RT_HIPd({"fr_in": T.frame(0, e, e, e, e, e, e), "t_name": pla, "tol": 1e-12, "t0": 1.0})
+    assert out.size == 0 and all(np.asarray(getattr(out, a)).size == 0 for a in ("x", "y", "z", "dx", "dy", "dz"))
+    one = T.frame(1, np.zeros(1), np.zeros(1), np.ones(1), np.zeros(1), np.zeros(1), -np.ones(1))
+    hit = B.RT_HIPd({"fr_in": one, "t_name": pla, "tol": 1e-12, "t0": 1.0})
+    np.testing.assert_allclose([hit.z[0], hit.dz[0]], [0.0, 1.0], atol=1e-12)
