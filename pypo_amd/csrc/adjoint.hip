@@ -390,7 +390,7 @@ __global__ __launch_bounds__(256) void k_adj_reverse64(const TgtRec64 *__restric
             double rev = __builtin_fma(pz, r.uz, __builtin_fma(py, r.uy, px * r.ux));
             double f = __builtin_amdgcn_fract(rev);
             double sn, cs;
-            sincospi(2.0 * f, &sn, &cs);
+            sincos_rev_f64(f, sn, cs);
             double w = __builtin_fma(nz, r.rz, __builtin_fma(ny, r.ry, nx * r.rx));
             cs = (w < 0.0) ? 0.0 : cs;
             sn = (w < 0.0) ? 0.0 : sn;
@@ -465,7 +465,7 @@ __global__ __launch_bounds__(256) void k_adj_currents(const SrcRec64 *__restrict
             const double d2r = -6.0 * tdir * q * dq, d2i = (1.0 - 9.0 * q2) * dq;
             const double d3r = -2.0 * q * dq, d3i = tdir * dq;
             double sn, cs;
-            sincospi(2.0 * __builtin_amdgcn_fract(R * rev_scale), &sn, &cs);   // G = exp(j t k R)
+            sincos_rev_f64(__builtin_amdgcn_fract(R * rev_scale), sn, cs);   // G = exp(j t k R)
             // dG = j t k nu G
             const double tkn = tdir * k * nu;
             const double dgr = -tkn * sn, dgi = tkn * cs;

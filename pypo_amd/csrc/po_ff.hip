@@ -118,7 +118,7 @@ __device__ __forceinline__ void ff_tile_fp64(const SrcRec64 *tile, const double 
             double rev = __builtin_fma(r.z, rz[j], __builtin_fma(r.y, ry[j], r.x * rx[j]));
             double f = __builtin_amdgcn_fract(rev);
             double sn, cs;
-            sincospi(2.0 * f, &sn, &cs);
+            sincos_rev_f64(f, sn, cs);
             if (MASK) {
                 double m = __builtin_fma(r.nz, uz[j], __builtin_fma(r.ny, uy[j], r.nx * ux[j]));
                 cs = (m < 0.0) ? 0.0 : cs;

@@ -95,7 +95,7 @@ __device__ __forceinline__ void near_tile_fp64(const SrcRec64 *tile, const doubl
             double Rinv = 1.0 / R;
             double f = __builtin_amdgcn_fract(R * rev_scale);
             double sn, cs;
-            sincospi(2.0 * f, &sn, &cs);
+            sincos_rev_f64(f, sn, cs);
             const double h[3] = {dx * Rinv, dy * Rinv, dz * Rinv};
             double m = __builtin_fma(r.nz, h[2], __builtin_fma(r.ny, h[1], r.nx * h[0]));
             cs = (m < 0.0) ? 0.0 : cs;

@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_scalar_fp64(const SrcRecS64 *__restrict
                 double R = sqrt(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)));
                 double f = __builtin_amdgcn_fract(R * rev_scale);
                 double sn, cs;
-                sincospi(2.0 * f, &sn, &cs);
+                sincos_rev_f64(f, sn, cs);
                 double Rinv = 1.0 / R;
                 double ur = r.ur * Rinv, ui = r.ui * Rinv;
                 ar[j] = __builtin_fma(ur, cs, ar[j]);
