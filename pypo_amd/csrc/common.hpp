@@ -130,31 +130,30 @@ __device__ __forceinline__ void sincos_rev_f32(float f, float &s, float &c) {
 
 // sin and cos of 2*pi*f in FP64 for a phase |f| <= 1 revolution (the FP64
 // kernels pass fract(phase)).  Quadrant n = rint(4 f), r = f - n / 4 (exact,
-// |r| <= 1/8); Taylor polynomials in r of sin(2 pi r) to r^17 and cos(2 pi r)
-// to r^16, coefficients (-1)^k (2 pi)^m / m! rounded to double (truncation
-// below 1e-19); then the quadrant's swap and signs.  No special cases: the
-// argument is a finite fraction by construction.
+// |r| <= 1/8); sin(2 pi r) = r P(r^2) with P of degree 6 and cos(2 pi r) =
+// Q(r^2) with Q of degree 7, both interpolating at Chebyshev nodes of r^2 in
+// [0, 1/64] (fitted in 50-digit arithmetic; with the coefficients rounded to
+// double the polynomials are within 2.6e-17 and 1.5e-17 of sin and cos); then
+// the quadrant's swap and signs.  No special cases: the argument is a finite
+// fraction by construction.
 __device__ __forceinline__ void sincos_rev_f64(double f, double &s, double &c) {
     const double n = __builtin_rint(4.0 * f);
     const double r = __builtin_fma(n, -0.25, f);
     const double r2 = r * r;
-    double ps = 0x1.aaec32af93359p-4;                 // (2 pi)^17 / 17!
-    ps = __builtin_fma(ps, r2, -0x1.6fadb9f155744p-1);
-    ps = __builtin_fma(ps, r2, 0x1.e8f434d018d63p+1);
-    ps = __builtin_fma(ps, r2, -0x1.e3074fde8871fp+3);
-    ps = __builtin_fma(ps, r2, 0x1.50783487ee782p+5);
-    ps = __builtin_fma(ps, r2, -0x1.32d2cce62bd86p+6);
-    ps = __builtin_fma(ps, r2, 0x1.466bc6775aae2p+6);
-    ps = __builtin_fma(ps, r2, -0x1.4abbce625be53p+5);
+    double ps = 0x1.e3f38399551bfp+1;
+    ps = __builtin_fma(ps, r2, -0x1.e30071afc3e59p+3);
+    ps = __builtin_fma(ps, r2, 0x1.50782fda12d96p+5);
+    ps = __builtin_fma(ps, r2, -0x1.32d2cce2e5b19p+6);
+    ps = __builtin_fma(ps, r2, 0x1.466bc677587f8p+6);
+    ps = __builtin_fma(ps, r2, -0x1.4abbce625be41p+5);
     ps = __builtin_fma(ps, r2, 0x1.921fb54442d18p+2);   // 2 pi
     ps *= r;
-    double pc = 0x1.20c62c2f2d7f5p-2;                 // (2 pi)^16 / 16!
-    pc = __builtin_fma(pc, r2, -0x1.b6e24f44b128fp+0);
-    pc = __builtin_fma(pc, r2, 0x1.f9d38a3763cc3p+2);
-    pc = __builtin_fma(pc, r2, -0x1.a6d1f2a204a8cp+4);
-    pc = __builtin_fma(pc, r2, 0x1.e1f506891babbp+5);
-    pc = __builtin_fma(pc, r2, -0x1.55d3c7e3cbffap+6);
-    pc = __builtin_fma(pc, r2, 0x1.03c1f081b5ac4p+6);
+    double pc = -0x1.b264ba152378ap+0;
+    pc = __builtin_fma(pc, r2, 0x1.f9cc41140bb60p+2);
+    pc = __builtin_fma(pc, r2, -0x1.a6d1ec7906c20p+4);
+    pc = __builtin_fma(pc, r2, 0x1.e1f5068355e15p+5);
+    pc = __builtin_fma(pc, r2, -0x1.55d3c7e3c90f8p+6);
+    pc = __builtin_fma(pc, r2, 0x1.03c1f081b5aacp+6);
     pc = __builtin_fma(pc, r2, -0x1.3bd3cc9be45dep+4);
     pc = __builtin_fma(pc, r2, 1.0);
     const int q = (int)n & 3;   // sin(2 pi (r + q/4)), cos(2 pi (r + q/4))
