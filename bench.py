@@ -311,7 +311,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the PO sums shard over targets with no exchange, "
-              "and this build runs on single-GPU machines; use --gpus 1", file=sys.stderr)
+              "but this benchmark runs on one GPU; use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch

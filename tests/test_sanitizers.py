@@ -5,8 +5,8 @@ entry point of the restatement on small cases, twice: with AddressSanitizer
 plus UndefinedBehaviorSanitizer, and with ThreadSanitizer (the targets and
 rays are split over std::threads as Propagation.h:1356-1377 and
 RayTrace.h:283-298 split them).  Any report ends the run with a non-zero
-status (-fno-sanitize-recover).  The driver also checks that one thread and
-three threads give bit-identical results."""
+status (-fno-sanitize-recover).  sanitize_main.cpp also checks that one
+thread and three threads give bit-identical results."""
 
 import os
 import shutil
