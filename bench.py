@@ -452,7 +452,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64",
+        "dtype": "fp64 phase + fp32 integrand, fp16 hi/lo MFMA products with fp32 accumulation",
         "data": "synthetic (seedless analytic Gaussian feed currents; grids generated on device)",
         "config": {"workload": "JM->FF large far field: feed 33x33 -> paraboloid D=3000 f=1200 uv "
                                "1001x1001 -> AoE 501x501 +-0.2deg, lambda=1mm",
