@@ -167,7 +167,7 @@ __device__ __forceinline__ void ff_kstep(const TileXYZ &tx, const half8 &b_hi, c
             f = (float)__builtin_amdgcn_fract(rev);
             if (FAST) f_lead = f;
         } else {
-            // |f| <= 1 + kLocalRev: in the hardware's own reduction range (see above)
+            // |f| <= kLocalRev (the bound counts f_lead): in the hardware's own reduction range
             f = __builtin_fmaf(dz[j], uz, __builtin_fmaf(dy[j], uy, __builtin_fmaf(dx[j], ux, f_lead)));
         }
         float sn, cs;
