@@ -303,3 +303,36 @@ def test_kernel_timing_api(hip):
         assert lib.pypo_hip_last_kernel_ms() > 0
     finally:
         lib.pypo_hip_set_timing(0)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_ff_matrix_random_cases(hip, port, seed):
+    """Seeded random far-field cases for the matrix kernel against the CPU
+    checker: xy or uv source grids of random shape (partial last tiles, row
+    lengths that do and do not divide by four), random wavelengths (follower
+    and FP64 phase tiles), random far-field cones from +-0.1 to +-80 deg,
+    random J and M currents, sometimes a tilted surface."""
+    rng = np.random.default_rng(1000 + seed)
+    nx, ny = (int(v) for v in rng.integers(3, 70, size=2))
+    if rng.random() < 0.5:
+        src = G.plane({"gmode": "xy", "lims_x": [-20, 20], "lims_y": [-15, 15], "gridsize": [nx, ny]})
+    else:
+        src = G.paraboloid({"pmode": "focus", "vertex": [0, 0, 0], "focus_1": [0, 0, 60.0], "gmode": "uv",
+                            "lims_u": [0, 25], "lims_v": [0, 360], "gridsize": [nx, ny]})
+    if rng.random() < 0.3:
+        src["transf"] = G.mat_translate([3.0, -2.0, 5.0], G.find_rotation([0, 0, 1], [0.2, 0.1, 1.0]))
+    deg = float(10 ** rng.uniform(-1, np.log10(80)))
+    ff = G.plane({"gmode": "AoE", "lims_Az": [-deg, deg], "lims_El": [-deg, deg],
+                  "gridsize": [int(rng.integers(3, 30)), int(rng.integers(3, 30))]})
+    lam = float(10 ** rng.uniform(np.log10(0.05), np.log10(5.0)))
+    k = 2 * np.pi / lam
+    comps = [rng.normal(size=(nx, ny)) + 1j * rng.normal(size=(nx, ny)) for _ in range(6)]
+    if rng.random() < 0.5:
+        comps[3:] = [np.zeros((nx, ny), complex)] * 3
+    cur = T.currents(*comps)
+    B.set_ff_kernel("matrix")
+    out = B.PyPO_HIPd(src, ff, {"mode": "FF", "k": k, "s_current": cur, "precision": "mixed"})
+    ref, _, _ = OB.propagate(port, "FF", src, ff, cur, k)
+    err = group_err(flat(out), flat(ref))
+    print(f"seed {seed}: {nx}x{ny} {src['gmode']} lam {lam:.3g} cone {deg:.3g} deg: {err:.2e}")
+    assert err <= TOL["mixed"]
