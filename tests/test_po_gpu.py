@@ -336,3 +336,33 @@ def test_ff_matrix_random_cases(hip, port, seed):
     err = group_err(flat(out), flat(ref))
     print(f"seed {seed}: {nx}x{ny} {src['gmode']} lam {lam:.3g} cone {deg:.3g} deg: {err:.2e}")
     assert err <= TOL["mixed"]
+
+
+@pytest.mark.parametrize("seed", range(15))
+def test_near_modes_random_cases(hip, port, seed):
+    """Seeded random near-field cases (JM, EH, JMEH, EHP, scalar by seed) in
+    mixed and FP64 precision against the CPU checker: random source and
+    target grid shapes (partial last tiles of every length, whose padding
+    records the sums skip), random wavelengths and distances."""
+    rng = np.random.default_rng(2000 + seed)
+    mode = ("JM", "EH", "JMEH", "EHP", "scalar")[seed % 5]
+    nx, ny = (int(v) for v in rng.integers(2, 40, size=2))
+    src = G.plane({"gmode": "xy", "lims_x": [-10, 10], "lims_y": [-8, 8], "gridsize": [nx, ny]})
+    tgt = G.paraboloid({"pmode": "focus", "vertex": [0, 0, 0], "focus_1": [0, 0, 80.0], "gmode": "uv",
+                        "lims_u": [0, 30], "lims_v": [0, 360],
+                        "gridsize": [int(rng.integers(3, 40)), int(rng.integers(3, 40))]})
+    tgt["transf"] = G.mat_translate([0, 0, float(rng.uniform(20, 200))])   # in front of the source plane
+    lam = float(10 ** rng.uniform(np.log10(0.2), np.log10(5.0)))
+    k = 2 * np.pi / lam
+    if mode == "scalar":
+        obj = T.scalarfield(rng.normal(size=(nx, ny)) + 1j * rng.normal(size=(nx, ny)))
+        key = "s_scalarfield"
+    else:
+        obj = T.currents(*[rng.normal(size=(nx, ny)) + 1j * rng.normal(size=(nx, ny)) for _ in range(6)])
+        key = "s_current"
+    ref, _, _ = OB.propagate(port, mode, src, tgt, obj, k)
+    for prec in ("mixed", "fp64"):
+        out = B.PyPO_HIPd(src, tgt, {"mode": mode, "k": k, key: obj, "precision": prec})
+        err = group_err(flat(out), flat(ref))
+        print(f"seed {seed} {mode} {prec}: src {nx}x{ny}, lam {lam:.3g}: {err:.2e}")
+        assert err <= TOL[prec]
