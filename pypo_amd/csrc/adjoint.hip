@@ -188,13 +188,14 @@ __global__ __launch_bounds__(256) void k_adj_tgt_meta(const TgtRec32 *__restrict
 struct RevTile {
     double u[3][TILE];       // r_t * rev_scale
     float r[3][TILE];        // r_t (shadow test)
+    float du[3][TILE];       // follower offsets u_t - u_leader (FP32 revolutions per mm)
+    int fast[4];             // per wave: 1 when all its followers may take the follower form
 };
 
 // This lane's target record (tile slot sl) -> its entries of both B blocks.
 __device__ __forceinline__ void build_rev_b(const TgtRec32 &r, int sl, float inv, mfma::half8 (*b0h)[64],
                                             mfma::half8 (*b0l)[64], mfma::half8 (*b1h)[64],
                                             mfma::half8 (*b1l)[64]) {
-    using mfma::half2v;
     const int ks = sl >> 4, g = (sl & 15) >> 2, jj = sl & 3;
     const float w[4] = {1.f, r.rx, r.ry, r.rz};
 #pragma unroll
@@ -205,44 +206,64 @@ __device__ __forceinline__ void build_rev_b(const TgtRec32 &r, int sl, float inv
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             const float re = r.a[2 * q] * w[v] * inv, im = r.a[2 * q + 1] * w[v] * inv;
-            half2v h, l;
-            mfma::split_pair(re, -im, h, l);
-            reinterpret_cast<half2v *>(&bh[ks][16 * g + c0 + 2 * q])[jj] = h;
-            reinterpret_cast<half2v *>(&bl[ks][16 * g + c0 + 2 * q])[jj] = l;
-            mfma::split_pair(im, re, h, l);
-            reinterpret_cast<half2v *>(&bh[ks][16 * g + c0 + 2 * q + 1])[jj] = h;
-            reinterpret_cast<half2v *>(&bl[ks][16 * g + c0 + 2 * q + 1])[jj] = l;
+            unsigned h, l;
+            mfma::split_pair_fast(re, -im, h, l);
+            reinterpret_cast<unsigned *>(&bh[ks][16 * g + c0 + 2 * q])[jj] = h;
+            reinterpret_cast<unsigned *>(&bl[ks][16 * g + c0 + 2 * q])[jj] = l;
+            mfma::split_pair_fast(im, re, h, l);
+            reinterpret_cast<unsigned *>(&bh[ks][16 * g + c0 + 2 * q + 1])[jj] = h;
+            reinterpret_cast<unsigned *>(&bl[ks][16 * g + c0 + 2 * q + 1])[jj] = l;
         }
     }
 }
 
-template <int MB, bool MASK>
+// One 256-target tile, 16 K-steps of 16 targets.  Phase as in the forward
+// matrix kernel, with the roles of the operands swapped: the targets a lane
+// takes per K-step are neighbours on the far-field grid, the second of them
+// (the leader) gets rev = p . u in FP64, the others (FAST tiles only)
+//   f_t = f_leader + p . (u_t - u_leader)       (three FP32 FMAs)
+// with |f_t| <= kLocalRev over the workgroup's dish points (checked when the
+// tile is staged).
+template <int MB, bool MASK, bool FAST>
 __device__ __forceinline__ void rev_tile_mfma(const RevTile &tt, const mfma::half8 (*b0h)[64],
                                               const mfma::half8 (*b0l)[64], const mfma::half8 (*b1h)[64],
                                               const mfma::half8 (*b1l)[64], int lane, const double (&px)[MB],
                                               const double (&py)[MB], const double (&pz)[MB],
-                                              const float (*nv)[64 * MB], int lt0, mfma::floatx4 (&acc0)[MB],
-                                              mfma::floatx4 (&acc1)[MB]) {
+                                              const float (&pxf)[MB], const float (&pyf)[MB],
+                                              const float (&pzf)[MB], const float (*nv)[64 * MB], int lt0,
+                                              mfma::floatx4 (&acc0)[MB], mfma::floatx4 (&acc1)[MB]) {
     using mfma::half8;
     const int kb = lane >> 4;
+    // j-th computation of an M-block: the leader (target 1) first
+    auto tgt_of = [](int j) { return FAST ? (j == 0 ? 1 : (j == 1 ? 0 : j)) : j; };
 #pragma unroll 1
     for (int ks = 0; ks < mfma::KSTEPS; ks++) {
         const half8 h0 = b0h[ks][lane], l0 = b0l[ks][lane], h1 = b1h[ks][lane], l1 = b1l[ks][lane];
         const int t0 = ks * 16 + kb * 4;
         double ux[4], uy[4], uz[4];
-        float rx[4], ry[4], rz[4];
+        float dx[4], dy[4], dz[4], rx[4], ry[4], rz[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            ux[j] = tt.u[0][t0 + j]; uy[j] = tt.u[1][t0 + j]; uz[j] = tt.u[2][t0 + j];
+            if (!FAST || j == 1) { ux[j] = tt.u[0][t0 + j]; uy[j] = tt.u[1][t0 + j]; uz[j] = tt.u[2][t0 + j]; }
+            if (FAST && j != 1) { dx[j] = tt.du[0][t0 + j]; dy[j] = tt.du[1][t0 + j]; dz[j] = tt.du[2][t0 + j]; }
             if (MASK) { rx[j] = tt.r[0][t0 + j]; ry[j] = tt.r[1][t0 + j]; rz[j] = tt.r[2][t0 + j]; }
         }
 #pragma unroll
         for (int m = 0; m < MB; m++) {
             unsigned ah[4], al[4];
+            float f_lead = 0.f;
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const double rev = __builtin_fma(pz[m], uz[j], __builtin_fma(py[m], uy[j], px[m] * ux[j]));
-                const float f = (float)__builtin_amdgcn_fract(rev);
+            for (int jj = 0; jj < 4; jj++) {
+                const int j = tgt_of(jj);
+                float f;
+                if (!FAST || j == 1) {
+                    const double rev = __builtin_fma(pz[m], uz[j], __builtin_fma(py[m], uy[j], px[m] * ux[j]));
+                    f = (float)__builtin_amdgcn_fract(rev);
+                    if (FAST) f_lead = f;
+                } else {
+                    // |f| <= kLocalRev (the bound counts f_lead): in the hardware's own reduction range
+                    f = __builtin_fmaf(pzf[m], dz[j], __builtin_fmaf(pyf[m], dy[j], __builtin_fmaf(pxf[m], dx[j], f_lead)));
+                }
                 float sn, cs;
                 sincos_rev_f32(f, sn, cs);
                 if (MASK) {
@@ -285,15 +306,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         b0h[ks][l] = half8{}; b0l[ks][l] = half8{}; b1h[ks][l] = half8{}; b1l[ks][l] = half8{};
     }
     double px[MB], py[MB], pz[MB];
+    float pxf[MB], pyf[MB], pzf[MB];
+    float pbx, pby, pbz;   // largest |p_x|, |p_y|, |p_z| over the workgroup's dish points
     Cone cone;
     {
         float fx[MB], fy[MB], fz[MB];
+        float mx = 0.f, my = 0.f, mz = 0.f;
         bool any = false;
 #pragma unroll
         for (int m = 0; m < MB; m++) {
             const int i = s0 + 16 * m + row;
             const bool valid = i < n_src;
             px[m] = valid ? src.x[i] : 0.0; py[m] = valid ? src.y[i] : 0.0; pz[m] = valid ? src.z[i] : 0.0;
+            pxf[m] = (float)px[m]; pyf[m] = (float)py[m]; pzf[m] = (float)pz[m];
+            mx = fmaxf(mx, fabsf(pxf[m])); my = fmaxf(my, fabsf(pyf[m])); mz = fmaxf(mz, fabsf(pzf[m]));
             fx[m] = valid ? (float)src.nx[i] : 0.f; fy[m] = valid ? (float)src.ny[i] : 0.f;
             fz[m] = valid ? (float)src.nz[i] : 1.f;
             const int lt = wave * 16 * MB + 16 * m + row;
@@ -308,6 +334,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 if (s0 + 16 * m + row < n_src) d = fminf(d, fx[m] * cone.ax + fy[m] * cone.ay + fz[m] * cone.az);
             cone.cos_half = fminf(cone.cos_half, block_min(d, scratch));
         }
+        pbx = -block_min(-mx, scratch); pby = -block_min(-my, scratch); pbz = -block_min(-mz, scratch);
     }
     floatx4 acc0[MB], acc1[MB];
     double tot0[MB][4], tot1[MB][4];
@@ -323,17 +350,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     for (int base = t_begin; base < t_end; base += TILE) {
         const TileMeta tm = tmeta[base / TILE];
         const float inv = mfma::tile_inv_scale(tm.cmax);
+        const bool valid = base + tid < t_end;
         TgtRec32 r{};
-        if (base + tid < t_end) r = tg[base + tid];
+        if (valid) r = tg[base + tid];
+        // offset from the group's leader (same wave); a slot past the end has
+        // zero B entries, so its phase only needs to stay finite: offset 0
+        const int ls = mfma::leader_slot(lane);
+        const double lux = __shfl(r.ux, ls, 64), luy = __shfl(r.uy, ls, 64), luz = __shfl(r.uz, ls, 64);
+        const float ddx = valid ? (float)(r.ux - lux) : 0.f;
+        const float ddy = valid ? (float)(r.uy - luy) : 0.f;
+        const float ddz = valid ? (float)(r.uz - luz) : 0.f;
+        const float bound = 1.f + fabsf(ddx) * pbx + fabsf(ddy) * pby + fabsf(ddz) * pbz;
+        const bool ok = (lane == ls) || (bound <= mfma::kLocalRev);   // NaN: not ok
+        const unsigned long long all_ok = __ballot(ok);
         __syncthreads();
         tt.u[0][tid] = r.ux; tt.u[1][tid] = r.uy; tt.u[2][tid] = r.uz;
         tt.r[0][tid] = r.rx; tt.r[1][tid] = r.ry; tt.r[2][tid] = r.rz;
+        tt.du[0][tid] = ddx; tt.du[1][tid] = ddy; tt.du[2][tid] = ddz;
+        if (lane == 0) tt.fast[wave] = all_ok == ~0ull;
         build_rev_b(r, tid, inv, b0h, b0l, b1h, b1l);
         __syncthreads();
-        if (cones_may_shadow(tm.ax, tm.ay, tm.az, tm.cos_half, cone))
-            rev_tile_mfma<MB, true>(tt, b0h, b0l, b1h, b1l, lane, px, py, pz, nv, wave * 16 * MB + row, acc0, acc1);
-        else
-            rev_tile_mfma<MB, false>(tt, b0h, b0l, b1h, b1l, lane, px, py, pz, nv, wave * 16 * MB + row, acc0, acc1);
+        const bool fast = __builtin_amdgcn_readfirstlane(tt.fast[0] & tt.fast[1] & tt.fast[2] & tt.fast[3]);
+        const int lt0 = wave * 16 * MB + row;
+        if (cones_may_shadow(tm.ax, tm.ay, tm.az, tm.cos_half, cone)) {
+            if (fast)
+                rev_tile_mfma<MB, true, true>(tt, b0h, b0l, b1h, b1l, lane, px, py, pz, pxf, pyf, pzf, nv, lt0, acc0, acc1);
+            else
+                rev_tile_mfma<MB, true, false>(tt, b0h, b0l, b1h, b1l, lane, px, py, pz, pxf, pyf, pzf, nv, lt0, acc0, acc1);
+        } else {
+            if (fast)
+                rev_tile_mfma<MB, false, true>(tt, b0h, b0l, b1h, b1l, lane, px, py, pz, pxf, pyf, pzf, nv, lt0, acc0, acc1);
+            else
+                rev_tile_mfma<MB, false, false>(tt, b0h, b0l, b1h, b1l, lane, px, py, pz, pxf, pyf, pzf, nv, lt0, acc0, acc1);
+        }
         const double scale = 1.0 / (double)inv;
 #pragma unroll
         for (int m = 0; m < MB; m++) {
@@ -714,8 +763,20 @@ int launch_displace(const GridPtrs &g0, const double *d, int n, const GridPtrs &
 
 namespace {
 
+// Compute units of the current device (a launch-shape hint; 256 if the
+// query fails).
+int device_cus() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+        return 256;
+    return n;
+}
+
 // Target splits of the reverse sum: enough workgroups to fill the chip,
-// at least four 256-target tiles per split.
+// at least four 256-target tiles per split.  The matrix kernel (two
+// workgroups per CU) picks the split count, up to 8, that minimises the
+// rounds of concurrent workgroups times the tiles each one runs.
 struct AdjPlan {
     bool matrix;   // reverse sum on the matrix cores (mixed precision)
     int rows;      // partial rows per split: 24 (matrix) or 12
@@ -736,6 +797,15 @@ AdjPlan adj_plan(int precision, int n_feed, int n_dish, int n_tgt) {
     p.rblocks = p.matrix ? (n_dish + TILE - 1) / TILE : (n_dish + TILE * p.spl - 1) / (TILE * p.spl);
     int splits = (2048 + p.rblocks - 1) / p.rblocks;
     const int max_splits = (n_tgt + 4 * TILE - 1) / (4 * TILE);
+    if (p.matrix) {
+        const long long slots = 2LL * device_cus();
+        const long long tiles = (n_tgt + TILE - 1) / TILE;
+        long long best = -1;
+        for (int s = 1; s <= std::min(8, std::max(1, max_splits)); s++) {
+            const long long cost = ((long long)p.rblocks * s + slots - 1) / slots * ((tiles + s - 1) / s);
+            if (best < 0 || cost < best) { best = cost; splits = s; }
+        }
+    }
     splits = std::max(1, std::min(splits, max_splits));
     p.chunk = ((n_tgt + splits - 1) / splits + TILE - 1) / TILE * TILE;
     p.splits = (n_tgt + p.chunk - 1) / p.chunk;

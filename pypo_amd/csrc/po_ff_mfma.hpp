@@ -37,13 +37,14 @@
 // over 2^24 points in [-9, 9]), so no v_fract_f32 is spent.  The FP32
 // rounding of that sum is at most a few ulps of its partial sums, which are
 // bounded per workgroup by 1 + sum_i |d_i| (|a_i| + sin(theta)) (a, theta:
-// the cone of the workgroup's target directions).  A K-step takes the
-// follower form only when that bound is <= kLocalRev for all its followers
-// (a wave-uniform flag per K-step, set when the tile is staged); otherwise
-// -- across a grid-row wrap, past the end of the sources, for wide target
-// cones on coarse grids -- all four sources get the FP64 phase.  The error
-// that stays is that of the FP32 direction u (2^-24 relative) times |d|,
-// a few 1e-7 revolutions for neighbours a few wavelengths apart.
+// the cone of the workgroup's target directions).  A 256-source tile takes
+// the follower form only when that bound is <= kLocalRev for all its
+// followers (one flag per tile, set when the tile is staged); otherwise
+// -- a grid-row wrap to a distant point, past the end of the sources, wide
+// target cones on coarse grids -- every source of the tile gets the FP64
+// phase.  The error that stays is that of the FP32 direction u (2^-24
+// relative) times |d|, a few 1e-7 revolutions for neighbours a few
+// wavelengths apart.
 //
 // Workgroup: 4 waves x MB M-blocks of 16 targets.  Per tile each lane takes
 // one FP32 source record from HBM (prefetched during the previous tile),
@@ -78,15 +79,10 @@ __device__ __forceinline__ float tile_inv_scale(float cmax) {
     return ldexpf(1.f, -e);
 }
 
-__device__ __forceinline__ void split_pair(float c, float s, half2v &hi, half2v &lo) {
-    const _Float16 hc = (_Float16)c, hs = (_Float16)s;
-    hi = half2v{hc, hs};
-    lo = half2v{(_Float16)(c - (float)hc), (_Float16)(s - (float)hs)};
-}
-
-// The same split for the per-pair A operand: one packed RNE conversion for
-// hi; c - hi and s - hi by v_fma_mix_f32 (hi read as FP16 halves, c and s
-// FP32: the differences are exact in FP32); one packed RNE conversion for lo.
+// Split of two FP32 values into packed FP16 hi and lo halves (x = hi + lo to
+// ~22 bits), for both operands: one packed RNE conversion for hi; c - hi and
+// s - hi by v_fma_mix_f32 (hi read as FP16 halves, c and s FP32: the
+// differences are exact in FP32); one packed RNE conversion for lo.
 // Issue cost per wave-instruction measured on gfx950 (4 waves per SIMD):
 // v_cvt_pk_f16_f32 5.4 cycles, v_fma_mix_f32 5.5, but v_fma_mix{lo,hi}_f16
 // 9.9 -- so this four-instruction form (21.8) beats cvt_pk + mixlo + mixhi
@@ -109,15 +105,15 @@ __device__ __forceinline__ void build_b(const SrcRec32 &r, int sl, float inv, ha
     for (int a = 0; a < 6; a++) {
         const float re = (a < 3 ? r.cj[2 * a] : r.cm[2 * (a - 3)]) * inv;
         const float im = (a < 3 ? r.cj[2 * a + 1] : r.cm[2 * (a - 3) + 1]) * inv;
-        half2v h, l;
+        unsigned h, l;
         // real column 2a: cos row Re, sin row -Im
-        split_pair(re, -im, h, l);
-        reinterpret_cast<half2v *>(&bhi[ks][16 * g + 2 * a])[jj] = h;
-        reinterpret_cast<half2v *>(&blo[ks][16 * g + 2 * a])[jj] = l;
+        split_pair_fast(re, -im, h, l);
+        reinterpret_cast<unsigned *>(&bhi[ks][16 * g + 2 * a])[jj] = h;
+        reinterpret_cast<unsigned *>(&blo[ks][16 * g + 2 * a])[jj] = l;
         // imaginary column 2a+1: cos row Im, sin row Re
-        split_pair(im, re, h, l);
-        reinterpret_cast<half2v *>(&bhi[ks][16 * g + 2 * a + 1])[jj] = h;
-        reinterpret_cast<half2v *>(&blo[ks][16 * g + 2 * a + 1])[jj] = l;
+        split_pair_fast(im, re, h, l);
+        reinterpret_cast<unsigned *>(&bhi[ks][16 * g + 2 * a + 1])[jj] = h;
+        reinterpret_cast<unsigned *>(&blo[ks][16 * g + 2 * a + 1])[jj] = l;
     }
 }
 

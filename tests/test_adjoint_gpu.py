@@ -4,6 +4,9 @@ pypo_hip_ff_power_grad_dev, pypo_amd.adjoint.FarFieldPower):
 * small case: displaced grid, forward JM -> FF and the gradient against the
   CPU checker (oracle/adjoint.py, itself pinned against finite differences
   of the C port): FP64 to 1e-10, mixed to 1e-5;
+* the mixed reverse sum on the matrix cores and on the vector ALUs, with
+  the matrix kernel's follower phase on some tiles (+-6 deg) and on none
+  (+-70 deg, shadowed pairs);
 * the autograd wrapper gives the same gradient as the explicit call;
 * full size (601x601 dish, D = 3 m, lambda = 1 mm, 36 Zernike modes of
   RMS 20 um, 201x201 far field): the mixed-precision gradient against
@@ -72,6 +75,33 @@ def test_gradient_matches_checker(small, precision, tol):
     assert rel <= tol and np.abs(got - ref).max() <= tol * np.abs(ref).max()
 
 
+def _reverse_follower_share(dish_xyz, th, ph, k):
+    """Share of (256-point workgroup, 256-target tile) pairs that the matrix
+    reverse sum runs in its follower-phase form: per group of four targets one
+    FP64 phase (the second target), FP32 offsets d = u_t - u_leader for the
+    other three (u = r k / 2 pi), taken when 1 + sum_i |d_i| max|p_i| <= 8
+    over the workgroup's dish points p for every follower of the tile.  Slots
+    past the last target have d = 0."""
+    u = np.stack([np.cos(ph) * np.sin(th), np.sin(ph) * np.sin(th), np.cos(th)]) * k / (2 * np.pi)
+    n = u.shape[1]
+    npad = -(-n // 256) * 256
+    up = np.zeros((3, npad))
+    up[:, :n] = u
+    idx = np.arange(npad)
+    lead = (idx & ~3) | 1
+    d = np.abs(up - up[:, lead])
+    d[:, idx >= n] = 0
+    d[:, idx == lead] = 0
+    nd = dish_xyz.shape[1]
+    nw = -(-nd // 256)
+    pp = np.zeros((3, nw * 256))
+    pp[:, :nd] = np.abs(dish_xyz)
+    pw = pp.reshape(3, nw, 256).max(axis=2)
+    dt = d.reshape(3, npad // 256, 256)
+    fast = np.stack([(1 + pw.T @ dt[:, t, :]).max(axis=1) <= 8 for t in range(npad // 256)], axis=1)
+    return fast.mean()
+
+
 @pytest.mark.parametrize("ff_deg", [6.0, 70.0])
 def test_gradient_matrix_and_vector_reverse_sums(hip, port, ff_deg):
     """The mixed backward far-field sum on the matrix cores (24 output columns
@@ -80,6 +110,10 @@ def test_gradient_matrix_and_vector_reverse_sums(hip, port, ff_deg):
     the shadow test, which the cone test cannot skip there."""
     from pypo_amd import bind as B
     c = small_shape_case(port, ff_deg=ff_deg)
+    # +-6 deg: one workgroup of three takes the follower phase; +-70 deg: none
+    share = _reverse_follower_share(c["dr"][:3], c["ffr"][0], c["ffr"][1], c["k"])
+    print(f"ff +-{ff_deg} deg: follower-phase share {share:.3f}")
+    assert (0 < share < 1) if ff_deg < 10 else share == 0
     prob = PA.ShapeProblem(c["feed"], c["cur"], c["dish"], c["ff"], c["k"])
     d = torch.tensor(c["d"], dtype=torch.float64, device="cuda")
     E, _ = OA.forward(port, c["fr"], c["fcells"], c["fc"], c["dr"], c["dcells"], c["d"], c["ffr"], c["k"])
@@ -154,6 +188,10 @@ def _loss(prob, d, T, precision):
 def test_full_size_gradient_against_finite_differences(full):
     prob, d0, T = full
     assert prob.n_dish == 601 * 601 and prob.n_ff == 201 * 201
+    gf = prob.g_ff.cpu().numpy()
+    share = _reverse_follower_share(prob.g_dish0.cpu().numpy()[:3], gf[0], gf[1], prob.k)
+    print(f"reverse sum: follower-phase share {share:.3f}")
+    assert 0 < share < 1   # both phase forms
     d = d0.clone().requires_grad_(True)
     t = time.perf_counter()
     P = prob.power(d)
