@@ -763,16 +763,6 @@ int launch_displace(const GridPtrs &g0, const double *d, int n, const GridPtrs &
 
 namespace {
 
-// Compute units of the current device (a launch-shape hint; 256 if the
-// query fails).
-int device_cus() {
-    int dev = 0, n = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
-        return 256;
-    return n;
-}
-
 // Target splits of the reverse sum: enough workgroups to fill the chip,
 // at least four 256-target tiles per split.  The matrix kernel (two
 // workgroups per CU) picks the split count, up to 8, that minimises the
@@ -797,15 +787,9 @@ AdjPlan adj_plan(int precision, int n_feed, int n_dish, int n_tgt) {
     p.rblocks = p.matrix ? (n_dish + TILE - 1) / TILE : (n_dish + TILE * p.spl - 1) / (TILE * p.spl);
     int splits = (2048 + p.rblocks - 1) / p.rblocks;
     const int max_splits = (n_tgt + 4 * TILE - 1) / (4 * TILE);
-    if (p.matrix) {
-        const long long slots = 2LL * device_cus();
-        const long long tiles = (n_tgt + TILE - 1) / TILE;
-        long long best = -1;
-        for (int s = 1; s <= std::min(8, std::max(1, max_splits)); s++) {
-            const long long cost = ((long long)p.rblocks * s + slots - 1) / slots * ((tiles + s - 1) / s);
-            if (best < 0 || cost < best) { best = cost; splits = s; }
-        }
-    }
+    if (p.matrix)
+        splits = tail_splits(p.rblocks, (n_tgt + TILE - 1) / TILE, 1, std::min(8, std::max(1, max_splits)),
+                             2LL * device_cus());
     splits = std::max(1, std::min(splits, max_splits));
     p.chunk = ((n_tgt + splits - 1) / splits + TILE - 1) / TILE * TILE;
     p.splits = (n_tgt + p.chunk - 1) / p.chunk;

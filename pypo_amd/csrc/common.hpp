@@ -234,6 +234,13 @@ int set_sum_shape(int tpl, int want_wgs);   // 0, 0: built-in choice
 int set_ff_kernel(int kind);                // PYPO_FF_MATRIX or PYPO_FF_VECTOR
 int ff_kernel();
 size_t partial_bytes(int mode, const SumPlan &p, int n_tgt);
+// Compute units of the current device (a launch-shape hint; 256 if the
+// query fails).
+int device_cus();
+// Split count s in [lo, hi] that minimises the rounds of `slots` concurrent
+// workgroups over blocks x s workgroups times the tiles each one runs
+// (ceil(tiles / s)): the idle tail of the last round.  Ties: fewer splits.
+int tail_splits(long long blocks, long long tiles, int lo, int hi, long long slots);
 
 // One PO propagation from packed sources to the target grid ct.  `res`
 // points to a device-side bundle struct of the mode's result type
