@@ -148,6 +148,17 @@ int launch_ff_sum(int precision, const SumPlan &p, const void *packed, int n_src
                   int n_tgt, double rev_scale, double *partial, hipStream_t st);
 int launch_ff_mfma(const SumPlan &p, const void *packed, int n_src, const GridPtrs &ct, int n_tgt,
                    double rev_scale, double *partial, hipStream_t st);
+// Workgroups of 256 lanes one CU holds at once for the far-field sum kernel
+// that launch_ff_sum / launch_ff_mfma pick for (precision, targets per lane).
+int ff_wgs_per_cu(int precision, int tpl);
+int ff_mfma_wgs_per_cu(int tpl);
+
+// Occupancy of a 256-lane kernel in workgroups per CU (1 if the query fails).
+inline int blocks_per_cu(const void *kernel) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, TILE, 0) != hipSuccess || n < 1) return 1;
+    return n;
+}
 int launch_ff_finish(const SumPlan &p, const GridPtrs &ct, int n_tgt, double tdir, double pref,
                      const c2Bundle &out, const double *partial, hipStream_t st);
 int launch_near_sum(int mode, int precision, const SumPlan &p, const void *packed, int n_src,

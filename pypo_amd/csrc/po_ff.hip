@@ -258,6 +258,16 @@ __global__ __launch_bounds__(256) void k_ff_finish(const double *__restrict__ pa
 
 }  // namespace
 
+int ff_wgs_per_cu(int precision, int tpl) {
+    if (precision != PYPO_PREC_FP64 && ff_kernel() == PYPO_FF_MATRIX) return ff_mfma_wgs_per_cu(tpl);
+    static const int f2 = blocks_per_cu(reinterpret_cast<const void *>(k_ff_fp64<2>));
+    static const int f1 = blocks_per_cu(reinterpret_cast<const void *>(k_ff_fp64<1>));
+    static const int m2 = blocks_per_cu(reinterpret_cast<const void *>(k_ff_mixed<2>));
+    static const int m1 = blocks_per_cu(reinterpret_cast<const void *>(k_ff_mixed<1>));
+    if (precision == PYPO_PREC_FP64) return tpl == 2 ? f2 : f1;
+    return tpl == 2 ? m2 : m1;
+}
+
 int launch_ff_sum(int precision, const SumPlan &p, const void *packed, int n_src, const GridPtrs &ct,
                   int n_tgt, double rev_scale, double *partial, hipStream_t st) {
     if (precision != PYPO_PREC_FP64 && ff_kernel() == PYPO_FF_MATRIX)

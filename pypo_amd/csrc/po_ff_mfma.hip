@@ -30,6 +30,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 }  // namespace
 
+int ff_mfma_wgs_per_cu(int tpl) {
+    static const int n8 = blocks_per_cu(reinterpret_cast<const void *>(k_ff_mfma8));
+    static const int n4 = blocks_per_cu(reinterpret_cast<const void *>(k_ff_mfma4));
+    return tpl == 2 ? n8 : n4;
+}
+
 int launch_ff_mfma(const SumPlan &p, const void *packed, int n_src, const GridPtrs &ct, int n_tgt,
                    double rev_scale, double *partial, hipStream_t st) {
     const TileMeta *meta = (const TileMeta *)((const char *)packed + meta_offset((int)sizeof(SrcRec32), n_src));

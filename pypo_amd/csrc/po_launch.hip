@@ -70,13 +70,15 @@ SumPlan plan_sum(int mode, int precision, int n_src, int n_tgt) {
     int max_splits = (n_src + 8 * TILE - 1) / (8 * TILE);
     if (splits > max_splits) splits = max_splits;
     if (splits < 1) splits = 1;
-    // The matrix far-field kernel holds two workgroups per CU: between half
-    // and twice that split count, take the one with the shortest idle tail
-    // (large FF, 251 001 x 1 002 001: 17 -> 25 splits, 95.0 -> 93.7 ms).
-    if (mode == PYPO_MODE_FF && precision != PYPO_PREC_FP64 && ff_kernel() == PYPO_FF_MATRIX &&
-        !g_wgs_override)
+    // Far field: between half and twice that split count, the one whose last
+    // round of concurrent workgroups (the kernel's occupancy x CUs) idles
+    // least.  Large FF, 251 001 x 1 002 001 (two workgroups per CU): 17 -> 25
+    // splits; matrix kernel 95.0 -> 93.7 ms, FP64 358.0 -> 352.9 ms, vector
+    // 125.0 -> 122.9 ms.
+    if (mode == PYPO_MODE_FF && !g_wgs_override)
         splits = tail_splits(p.tblocks, (n_src + TILE - 1) / TILE, std::max(1, splits / 2),
-                             std::min(max_splits, 2 * splits), 2LL * device_cus());
+                             std::min(max_splits, 2 * splits),
+                             (long long)ff_wgs_per_cu(precision, p.tpl) * device_cus());
     int chunk = (n_src + splits - 1) / splits;
     chunk = (chunk + TILE - 1) / TILE * TILE;
     p.chunk = chunk;
