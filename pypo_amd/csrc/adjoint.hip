@@ -467,11 +467,16 @@ __global__ __launch_bounds__(256) void k_adj_reverse64(const TgtRec64 *__restric
 // Incident H at each dish point and its directional derivative along the dish
 // normal, from the feed's packed FP64 records (w*A*J, w*A*M): fieldAtPoint's
 // H integrand (Propagation.h:1084-1119) and its derivative with respect to the
-// target position moving along n.  Output rows [12][n]: J = 2 pref n x H
-// (re xyz, im xyz), then dJ = 2 pref n x dH.  FP64 throughout.
+// target position moving along n.  Grid (dish blocks, feed splits of
+// fchunk points); output rows [split][12][n]: J = 2 pref n x H (re xyz,
+// im xyz), then dJ = 2 pref n x dH, each split's share of the feed sum (the
+// combine kernels add the splits).  FP64 throughout.
 __global__ __launch_bounds__(256) void k_adj_currents(const SrcRec64 *__restrict__ feed, int n_feed,
-                                                GridPtrs dish, int n_dish, double k, double tdir,
-                                                double pref, double *__restrict__ jd) {
+                                                      int fchunk, GridPtrs dish, int n_dish, double k,
+                                                      double tdir, double pref, double *__restrict__ jd) {
+    // feed range of this split; its own 12 rows of jd
+    const int f0 = blockIdx.y * fchunk, f1 = min(n_feed, f0 + fchunk);
+    jd += (size_t)blockIdx.y * 12 * n_dish;
     __shared__ SrcRec64 tile[TILE];
     const int tid = threadIdx.x;
     const int i = blockIdx.x * TILE + tid;
@@ -482,10 +487,10 @@ __global__ __launch_bounds__(256) void k_adj_currents(const SrcRec64 *__restrict
     double hf[6] = {0, 0, 0, 0, 0, 0};   // H: re xyz, im xyz
     double dh[6] = {0, 0, 0, 0, 0, 0};   // dH
     const double rev_scale = tdir * k / kTwoPi;
-    for (int base = 0; base < n_feed; base += TILE) {
+    for (int base = f0; base < f1; base += TILE) {
         __syncthreads();
         const int s = base + tid;
-        if (s < n_feed) {
+        if (s < f1) {
             tile[tid] = feed[s];
         } else {
             SrcRec64 z{};
@@ -493,7 +498,7 @@ __global__ __launch_bounds__(256) void k_adj_currents(const SrcRec64 *__restrict
             tile[tid] = z;
         }
         __syncthreads();
-        const int cnt = min(TILE, n_feed - base);
+        const int cnt = min(TILE, f1 - base);
 #pragma unroll 1
         for (int s2 = 0; s2 < cnt; s2++) {
             const SrcRec64 &r = tile[s2];
@@ -572,8 +577,11 @@ __global__ __launch_bounds__(256) void k_adj_currents(const SrcRec64 *__restrict
 // FP32 unit vectors, s-coefficients, their derivatives and the per-tile
 // sums (folded into FP64).  Feed records are the mixed-precision SrcRec32.
 __global__ __launch_bounds__(256) void k_adj_currents32(const SrcRec32 *__restrict__ feed, int n_feed,
-                                                        GridPtrs dish, int n_dish, double k, double tdir,
-                                                        double pref, double *__restrict__ jd) {
+                                                        int fchunk, GridPtrs dish, int n_dish, double k,
+                                                        double tdir, double pref, double *__restrict__ jd) {
+    // feed range of this split; its own 12 rows of jd
+    const int f0 = blockIdx.y * fchunk, f1 = min(n_feed, f0 + fchunk);
+    jd += (size_t)blockIdx.y * 12 * n_dish;
     __shared__ SrcRec32 tile[TILE];
     const int tid = threadIdx.x;
     const int i = blockIdx.x * TILE + tid;
@@ -588,10 +596,10 @@ __global__ __launch_bounds__(256) void k_adj_currents32(const SrcRec32 *__restri
     double HF[6], DH[6];     // totals
 #pragma unroll
     for (int q = 0; q < 6; q++) { hf[q] = 0.f; dh[q] = 0.f; HF[q] = 0.0; DH[q] = 0.0; }
-    for (int base = 0; base < n_feed; base += TILE) {
+    for (int base = f0; base < f1; base += TILE) {
         __syncthreads();
         const int s = base + tid;
-        if (s < n_feed) {
+        if (s < f1) {
             tile[tid] = feed[s];
         } else {
             SrcRec32 z{};
@@ -599,7 +607,7 @@ __global__ __launch_bounds__(256) void k_adj_currents32(const SrcRec32 *__restri
             tile[tid] = z;
         }
         __syncthreads();
-        const int cnt = min(TILE, n_feed - base);
+        const int cnt = min(TILE, f1 - base);
 #pragma unroll 1
         for (int s2 = 0; s2 < cnt; s2++) {
             const SrcRec32 &r = tile[s2];
@@ -678,25 +686,36 @@ __global__ __launch_bounds__(256) void k_adj_currents32(const SrcRec32 *__restri
 }
 
 // dL/dd_i = c_i [ Re(B_i . dJ_i) + t k Im(J_i . G_i) ]
+// The 12 rows of J and dJ/dd at point i, summed over the feed splits (the
+// first split's twelve loads issue together).
+__device__ __forceinline__ void load_jd(const double *__restrict__ jd, int jsplits, int n, int i, double (&v)[12]) {
+#pragma unroll
+    for (int q = 0; q < 12; q++) v[q] = jd[(size_t)q * n + i];
+    for (int s = 1; s < jsplits; s++)
+#pragma unroll
+        for (int q = 0; q < 12; q++) v[q] += jd[((size_t)s * 12 + q) * n + i];
+}
+
 __global__ __launch_bounds__(256) void k_adj_combine(const double *__restrict__ partial, int splits,
                                                      GridPtrs dish, int n, int gmode, int ncx,
                                                      int ncy, const double *__restrict__ jd,
-                                                     double k, double tdir, double *__restrict__ grad) {
+                                                     int jsplits, double k, double tdir,
+                                                     double *__restrict__ grad) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double S[12];
+    double S[12];   // split sums, the first split's loads issued together
 #pragma unroll
-    for (int q = 0; q < 12; q++) {
-        double v = 0.0;
-        for (int s = 0; s < splits; s++) v += partial[((size_t)s * 12 + q) * n + i];
-        S[q] = v;
-    }
+    for (int q = 0; q < 12; q++) S[q] = partial[(size_t)q * n + i];
+    for (int s = 1; s < splits; s++)
+#pragma unroll
+        for (int q = 0; q < 12; q++) S[q] += partial[((size_t)s * 12 + q) * n + i];
+    double JD[12];
+    load_jd(jd, jsplits, n, i, JD);
     double re_bdj = 0.0, im_jg = 0.0;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const double br = S[2 * a], bi = S[2 * a + 1], gr = S[6 + 2 * a], gi = S[7 + 2 * a];
-        const double jr = jd[(size_t)a * n + i], ji = jd[(size_t)(3 + a) * n + i];
-        const double dr = jd[(size_t)(6 + a) * n + i], di = jd[(size_t)(9 + a) * n + i];
+        const double jr = JD[a], ji = JD[3 + a], dr = JD[6 + a], di = JD[9 + a];
         re_bdj += br * dr - bi * di;
         im_jg += jr * gi + ji * gr;
     }
@@ -707,18 +726,19 @@ __global__ __launch_bounds__(256) void k_adj_combine(const double *__restrict__ 
 // The 24-row layout of k_adj_reverse_mfma: G = n_x G_x + n_y G_y + n_z G_z in FP64.
 __global__ __launch_bounds__(256) void k_adj_combine24(const double *__restrict__ partial, int splits,
                                                        GridPtrs dish, int n, int gmode, int ncx, int ncy,
-                                                       const double *__restrict__ jd, double k, double tdir,
-                                                       double *__restrict__ grad) {
+                                                       const double *__restrict__ jd, int jsplits, double k,
+                                                       double tdir, double *__restrict__ grad) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double S[24];
+    double S[24];   // split sums, the first split's loads issued together
 #pragma unroll
-    for (int q = 0; q < 24; q++) {
-        double v = 0.0;
-        for (int s = 0; s < splits; s++) v += partial[((size_t)s * 24 + q) * n + i];
-        S[q] = v;
-    }
+    for (int q = 0; q < 24; q++) S[q] = partial[(size_t)q * n + i];
+    for (int s = 1; s < splits; s++)
+#pragma unroll
+        for (int q = 0; q < 24; q++) S[q] += partial[((size_t)s * 24 + q) * n + i];
     const double nrm[3] = {dish.nx[i], dish.ny[i], dish.nz[i]};
+    double JD[12];
+    load_jd(jd, jsplits, n, i, JD);
     double re_bdj = 0.0, im_jg = 0.0;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -729,8 +749,7 @@ __global__ __launch_bounds__(256) void k_adj_combine24(const double *__restrict_
             gr += nrm[b] * S[6 + 6 * b + 2 * a];
             gi += nrm[b] * S[7 + 6 * b + 2 * a];
         }
-        const double jr = jd[(size_t)a * n + i], ji = jd[(size_t)(3 + a) * n + i];
-        const double dr = jd[(size_t)(6 + a) * n + i], di = jd[(size_t)(9 + a) * n + i];
+        const double jr = JD[a], ji = JD[3 + a], dr = JD[6 + a], di = JD[9 + a];
         re_bdj += br * dr - bi * di;
         im_jg += jr * gi + ji * gr;
     }
@@ -774,6 +793,7 @@ struct AdjPlan {
     int blocks;    // workgroups of one point per lane
     int rblocks;   // reverse-sum workgroups along the dish points
     int splits, chunk;
+    int fsplits, fchunk;   // feed splits of the currents pass (grid.y) and feed points per split
     size_t tg_bytes, feed_bytes, part_bytes, jd_bytes;
 };
 
@@ -793,12 +813,30 @@ AdjPlan adj_plan(int precision, int n_feed, int n_dish, int n_tgt) {
     splits = std::max(1, std::min(splits, max_splits));
     p.chunk = ((n_tgt + splits - 1) / splits + TILE - 1) / TILE * TILE;
     p.splits = (n_tgt + p.chunk - 1) / p.chunk;
+    // The currents pass runs one dish point per lane over all feed points:
+    // split the feed (up to 8 ways, whole tiles) when the dish blocks alone
+    // leave the last round of workgroups part-empty.  Only at low occupancy:
+    // at four or more workgroups per CU the waves of a part-empty last round
+    // share their SIMDs with fewer others and finish that much sooner (601^2
+    // dish, 33^2 feed: FP64 kernel, 2 per CU, 4.99 -> 4.31 ms with 5 splits;
+    // mixed kernel, 4 per CU, 2.53 -> 2.58 ms).
+    {
+        static const int occ64 = blocks_per_cu(reinterpret_cast<const void *>(k_adj_currents));
+        static const int occ32 = blocks_per_cu(reinterpret_cast<const void *>(k_adj_currents32));
+        const int ftiles = (n_feed + TILE - 1) / TILE;
+        const int occ = precision == PYPO_PREC_FP64 ? occ64 : occ32;
+        const int fs = occ >= 4 ? 1
+                                : tail_splits(p.blocks, ftiles, 1, std::max(1, std::min(8, ftiles)),
+                                              (long long)occ * device_cus());
+        p.fchunk = (ftiles + fs - 1) / fs * TILE;
+        p.fsplits = (n_feed + p.fchunk - 1) / p.fchunk;
+    }
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     p.tg_bytes = up((size_t)n_tgt * (precision == PYPO_PREC_FP64 ? sizeof(TgtRec64) : sizeof(TgtRec32))) +
                  up((size_t)((n_tgt + TILE - 1) / TILE) * sizeof(TileMeta));
     p.feed_bytes = up(packed_bytes(PYPO_MODE_JM, precision, n_feed));
     p.part_bytes = up((size_t)p.splits * p.rows * n_dish * sizeof(double));
-    p.jd_bytes = up((size_t)12 * n_dish * sizeof(double));
+    p.jd_bytes = up((size_t)p.fsplits * 12 * n_dish * sizeof(double));
     return p;
 }
 
@@ -840,11 +878,11 @@ int launch_ff_power_grad(int precision, const GridPtrs &dish, int n_dish, int gm
     int err = launch_pack_sources(precision, feed, n_feed, feed_cur, fgmode, fncx, fncy, fpk, st);
     if (err) return err;
     if (fp64)
-        hipLaunchKernelGGL(k_adj_currents, dim3(blocks), dim3(TILE), 0, st, (const SrcRec64 *)fpk, n_feed,
-                           dish, n_dish, k, tdir, pc.prefactor, jd);
+        hipLaunchKernelGGL(k_adj_currents, dim3(blocks, p.fsplits), dim3(TILE), 0, st, (const SrcRec64 *)fpk,
+                           n_feed, p.fchunk, dish, n_dish, k, tdir, pc.prefactor, jd);
     else
-        hipLaunchKernelGGL(k_adj_currents32, dim3(blocks), dim3(TILE), 0, st, (const SrcRec32 *)fpk,
-                           n_feed, dish, n_dish, k, tdir, pc.prefactor, jd);
+        hipLaunchKernelGGL(k_adj_currents32, dim3(blocks, p.fsplits), dim3(TILE), 0, st, (const SrcRec32 *)fpk,
+                           n_feed, p.fchunk, dish, n_dish, k, tdir, pc.prefactor, jd);
     PYPO_CHECK_LAUNCH();
     if (fp64) {
         hipLaunchKernelGGL(k_adj_reverse64, dim3(p.rblocks, splits), dim3(TILE), 0, st,
@@ -863,10 +901,10 @@ int launch_ff_power_grad(int precision, const GridPtrs &dish, int n_dish, int gm
     PYPO_CHECK_LAUNCH();
     if (p.matrix)
         hipLaunchKernelGGL(k_adj_combine24, dim3(blocks), dim3(256), 0, st, part, splits, dish, n_dish, gmode,
-                           ncx, ncy, jd, k, tdir, grad);
+                           ncx, ncy, jd, p.fsplits, k, tdir, grad);
     else
         hipLaunchKernelGGL(k_adj_combine, dim3(blocks), dim3(256), 0, st, part, splits, dish, n_dish, gmode,
-                           ncx, ncy, jd, k, tdir, grad);
+                           ncx, ncy, jd, p.fsplits, k, tdir, grad);
     PYPO_CHECK_LAUNCH();
     return 0;
 }
