@@ -64,6 +64,19 @@ class ShapeProblem:
         self.n_ff = self.g_ff.shape[1]
         self.dish_cells = (int(dish["gmode"]),) + tuple(int(v) for v in gridsize(dish))
         self.feed_cells = (int(feed["gmode"]),) + tuple(int(v) for v in gridsize(feed))
+        # The backward sum over far-field targets does not depend on their
+        # order.  Handed over row by row with every other row reversed, the
+        # matrix kernel's groups of four consecutive targets never straddle a
+        # jump back across the grid (a row wrap), so all of its tiles may take
+        # the follower phase.
+        nx, ny = (int(v) for v in gridsize(ff))
+        self._tperm = None
+        self._g_ff_rev = self.g_ff
+        if nx * ny == self.n_ff and nx > 1 and ny > 1:
+            idx = torch.arange(self.n_ff, device=self.g_ff.device).view(nx, ny)
+            idx[1::2] = idx[1::2].flip(1)
+            self._tperm = idx.reshape(-1)
+            self._g_ff_rev = self.g_ff[:, self._tperm].contiguous()
 
     def _prec(self, precision):
         prec = precision or self.precision
@@ -100,11 +113,16 @@ class ShapeProblem:
         prec = self._prec(precision)
         g = self.displaced(d)
         E = E.detach().contiguous()
+        if E.shape != (12, self.n_ff) or not E.is_cuda or E.dtype != torch.float64:
+            raise ValueError(f"E: expected a (12, {self.n_ff}) float64 CUDA tensor")
         gp = dl_dp.detach().to(torch.float64).contiguous()
         if gp.shape != (self.n_ff,) or not gp.is_cuda:
             raise ValueError(f"dL/dP: expected a ({self.n_ff},) CUDA tensor")
         grad = torch.empty(self.n_dish, dtype=torch.float64, device=g.device)
-        cd, cf, cff = D.container(g), D.container(self.g_feed), D.container(self.g_ff)
+        if self._tperm is not None:
+            E = E[:, self._tperm].contiguous()
+            gp = gp[self._tperm].contiguous()
+        cd, cf, cff = D.container(g), D.container(self.g_feed), D.container(self._g_ff_rev)
         bf, be = D.bundle(self.c_feed), D.bundle(E)
         gm, ncx, ncy = self.dish_cells
         fgm, fncx, fncy = self.feed_cells

@@ -350,11 +350,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     for (int base = t_begin; base < t_end; base += TILE) {
         const TileMeta tm = tmeta[base / TILE];
         const float inv = mfma::tile_inv_scale(tm.cmax);
+        // A slot past the end carries the last target's direction and no
+        // amplitude (zero B entries): its phase only has to stay finite, and
+        // a follower whose leader lies past the end keeps a small offset.
         const bool valid = base + tid < t_end;
-        TgtRec32 r{};
-        if (valid) r = tg[base + tid];
-        // offset from the group's leader (same wave); a slot past the end has
-        // zero B entries, so its phase only needs to stay finite: offset 0
+        TgtRec32 r = tg[valid ? base + tid : t_end - 1];
+        if (!valid) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) r.a[q] = 0.f;
+        }
+        // offset from the group's leader (same wave); 0 past the end
         const int ls = mfma::leader_slot(lane);
         const double lux = __shfl(r.ux, ls, 64), luy = __shfl(r.uy, ls, 64), luz = __shfl(r.uz, ls, 64);
         const float ddx = valid ? (float)(r.ux - lux) : 0.f;

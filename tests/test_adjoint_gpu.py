@@ -5,8 +5,8 @@ pypo_hip_ff_power_grad_dev, pypo_amd.adjoint.FarFieldPower):
   CPU checker (oracle/adjoint.py, itself pinned against finite differences
   of the C port): FP64 to 1e-10, mixed to 1e-5;
 * the mixed reverse sum on the matrix cores and on the vector ALUs, with
-  the matrix kernel's follower phase on some tiles (+-6 deg) and on none
-  (+-70 deg, shadowed pairs);
+  the matrix kernel's follower phase on every tile (+-6 deg) and on one
+  workgroup of three (+-70 deg, with shadowed pairs);
 * the autograd wrapper gives the same gradient as the explicit call;
 * full size (601x601 dish, D = 3 m, lambda = 1 mm, 36 Zernike modes of
   RMS 20 um, 201x201 far field): the mixed-precision gradient against
@@ -75,17 +75,25 @@ def test_gradient_matches_checker(small, precision, tol):
     assert rel <= tol and np.abs(got - ref).max() <= tol * np.abs(ref).max()
 
 
+def _serpentine(nx, ny):
+    """The order ShapeProblem hands the far-field targets to the backward
+    sum in: row by row, every other row reversed."""
+    idx = np.arange(nx * ny).reshape(nx, ny)
+    idx[1::2] = idx[1::2, ::-1]
+    return idx.reshape(-1)
+
+
 def _reverse_follower_share(dish_xyz, th, ph, k):
     """Share of (256-point workgroup, 256-target tile) pairs that the matrix
     reverse sum runs in its follower-phase form: per group of four targets one
     FP64 phase (the second target), FP32 offsets d = u_t - u_leader for the
     other three (u = r k / 2 pi), taken when 1 + sum_i |d_i| max|p_i| <= 8
     over the workgroup's dish points p for every follower of the tile.  Slots
-    past the last target have d = 0."""
+    past the last target carry its direction and have d = 0."""
     u = np.stack([np.cos(ph) * np.sin(th), np.sin(ph) * np.sin(th), np.cos(th)]) * k / (2 * np.pi)
     n = u.shape[1]
     npad = -(-n // 256) * 256
-    up = np.zeros((3, npad))
+    up = np.repeat(u[:, -1:], npad, axis=1)
     up[:, :n] = u
     idx = np.arange(npad)
     lead = (idx & ~3) | 1
@@ -110,10 +118,11 @@ def test_gradient_matrix_and_vector_reverse_sums(hip, port, ff_deg):
     the shadow test, which the cone test cannot skip there."""
     from pypo_amd import bind as B
     c = small_shape_case(port, ff_deg=ff_deg)
-    # +-6 deg: one workgroup of three takes the follower phase; +-70 deg: none
-    share = _reverse_follower_share(c["dr"][:3], c["ffr"][0], c["ffr"][1], c["k"])
+    # +-6 deg: every tile takes the follower phase; +-70 deg: one workgroup of three
+    perm = _serpentine(*map(int, c["ff"]["gridsize"]))
+    share = _reverse_follower_share(c["dr"][:3], c["ffr"][0][perm], c["ffr"][1][perm], c["k"])
     print(f"ff +-{ff_deg} deg: follower-phase share {share:.3f}")
-    assert (0 < share < 1) if ff_deg < 10 else share == 0
+    assert (share == 1) if ff_deg < 10 else 0 < share < 1
     prob = PA.ShapeProblem(c["feed"], c["cur"], c["dish"], c["ff"], c["k"])
     d = torch.tensor(c["d"], dtype=torch.float64, device="cuda")
     E, _ = OA.forward(port, c["fr"], c["fcells"], c["fc"], c["dr"], c["dcells"], c["d"], c["ffr"], c["k"])
@@ -158,6 +167,8 @@ def test_bad_arguments_raise(small):
         prob.displaced(d.float())
     with pytest.raises(ValueError):
         prob.power_grad(d, prob.far_field(d), torch.zeros(3, dtype=torch.float64, device="cuda"))
+    with pytest.raises(ValueError):   # far field of the wrong length
+        prob.power_grad(d, prob.far_field(d)[:, :-1], torch.zeros(prob.n_ff, dtype=torch.float64, device="cuda"))
     lib = B.loadHIPlib()
     # NULL arrays and an unknown precision are rejected before any launch
     assert lib.pypo_hip_ff_power_grad_dev(None, 1, 1, 1, None, 0, 1, 1, None, None, None, None, 1.0, -1.0,
@@ -189,9 +200,11 @@ def test_full_size_gradient_against_finite_differences(full):
     prob, d0, T = full
     assert prob.n_dish == 601 * 601 and prob.n_ff == 201 * 201
     gf = prob.g_ff.cpu().numpy()
-    share = _reverse_follower_share(prob.g_dish0.cpu().numpy()[:3], gf[0], gf[1], prob.k)
-    print(f"reverse sum: follower-phase share {share:.3f}")
-    assert 0 < share < 1   # both phase forms
+    perm = _serpentine(201, 201)
+    share = _reverse_follower_share(prob.g_dish0.cpu().numpy()[:3], gf[0][perm], gf[1][perm], prob.k)
+    share_rows = _reverse_follower_share(prob.g_dish0.cpu().numpy()[:3], gf[0], gf[1], prob.k)
+    print(f"reverse sum: follower-phase share {share:.3f} (row by row: {share_rows:.3f})")
+    assert share == 1 and share_rows < 1
     d = d0.clone().requires_grad_(True)
     t = time.perf_counter()
     P = prob.power(d)
