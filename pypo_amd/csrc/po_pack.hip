@@ -7,11 +7,24 @@
 // unless the grid is uv) and A the area element.  A second pass writes one
 // TileMeta per 256 records (normal cone, any non-zero M).  HBM-bound:
 // 152 B read and 96 B (FP32 records) or 144 B (FP64) written per source.
+//
+// Records go out in serpentine order (grid rows alternately reversed): the
+// sums do not depend on the source order, and this way consecutive records
+// are grid neighbours across row ends too, which the matrix far field's
+// follower phase needs for groups of four that span a row end.
 
 #include "po_common.hpp"
 
 namespace pypo {
 namespace {
+
+// Record slot of grid point i (i = xu ncy + yv): odd rows reversed when the
+// grid is ncx x ncy, else i.
+__device__ __forceinline__ int packed_slot(int i, int n, int ncx, int ncy) {
+    if (ncy < 2 || (long long)ncx * ncy != n) return i;
+    const int xu = i / ncy, yv = i - xu * ncy;
+    return (xu & 1) ? xu * ncy + (ncy - 1 - yv) : i;
+}
 
 __global__ __launch_bounds__(256) void k_pack32(GridPtrs cs, int n, Bundle12 cur, int gmode,
                                                 int ncx, int ncy, SrcRec32 *__restrict__ out) {
@@ -29,7 +42,7 @@ __global__ __launch_bounds__(256) void k_pack32(GridPtrs cs, int n, Bundle12 cur
         r.cm[2 * a + 1] = (float)(wa * cur.p[9 + a][i]);   // M im
     }
     r.pad1[0] = 0.f; r.pad1[1] = 0.f;
-    out[i] = r;
+    out[packed_slot(i, n, ncx, ncy)] = r;
 }
 
 __global__ __launch_bounds__(256) void k_pack64(GridPtrs cs, int n, Bundle12 cur, int gmode,
@@ -47,7 +60,7 @@ __global__ __launch_bounds__(256) void k_pack64(GridPtrs cs, int n, Bundle12 cur
         r.c[6 + 2 * a] = wa * cur.p[6 + a][i];
         r.c[7 + 2 * a] = wa * cur.p[9 + a][i];
     }
-    out[i] = r;
+    out[packed_slot(i, n, ncx, ncy)] = r;
 }
 
 __global__ __launch_bounds__(256) void k_pack_s32(GridPtrs cs, int n, const double *__restrict__ ur,
@@ -59,7 +72,7 @@ __global__ __launch_bounds__(256) void k_pack_s32(GridPtrs cs, int n, const doub
     SrcRecS32 r;
     r.x = cs.x[i]; r.y = cs.y[i]; r.z = cs.z[i];
     r.ur = (float)(wa * ur[i]); r.ui = (float)(wa * ui[i]);
-    out[i] = r;
+    out[packed_slot(i, n, ncx, ncy)] = r;
 }
 
 __global__ __launch_bounds__(256) void k_pack_s64(GridPtrs cs, int n, const double *__restrict__ ur,
@@ -71,7 +84,7 @@ __global__ __launch_bounds__(256) void k_pack_s64(GridPtrs cs, int n, const doub
     SrcRecS64 r;
     r.x = cs.x[i]; r.y = cs.y[i]; r.z = cs.z[i];
     r.ur = wa * ur[i]; r.ui = wa * ui[i]; r.pad = 0.0;
-    out[i] = r;
+    out[packed_slot(i, n, ncx, ncy)] = r;
 }
 
 template <class Rec>

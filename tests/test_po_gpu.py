@@ -114,7 +114,11 @@ def _follower_tiles(src_grid, th, ph, k):
     a /= np.linalg.norm(a)
     cos_half = np.min(a @ u)
     ub = np.minimum(1.0, np.abs(a) + np.sqrt(max(0.0, 1 - cos_half ** 2)) + 1e-3) if cos_half > 0 else np.ones(3)
-    r = np.stack([src_grid.x, src_grid.y, src_grid.z]).reshape(3, -1)
+    # the packed records are in serpentine order: odd grid rows reversed
+    r = np.stack([src_grid.x, src_grid.y, src_grid.z]).reshape(3, *np.shape(src_grid.x))
+    if r.ndim == 3:
+        r[:, 1::2] = r[:, 1::2, ::-1].copy()
+    r = r.reshape(3, -1)
     n = r.shape[1]
     flags = []
     for t0 in range(0, n, 256):
@@ -141,11 +145,13 @@ def test_ff_matrix_phase_forms(hip, port, case):
     tiles but with offsets d.u up to 5 revolutions (phases up to ~6 handed to
     v_sin/v_cos unreduced); the same plane at lambda = 0.05 mm
     (neighbours 10 revolutions apart: every tile takes the full FP64 phase);
-    and 16 x 301 points at lambda = 2 mm, where the groups of four that
-    straddle a row end (jumping across the plane) put the tiles holding them
-    on the FP64 form and the others on the follower form."""
+    and 16 x 301 points at lambda = 0.2 mm, where neighbours along a row
+    are 0.53 revolutions apart but the rows 10.7: the groups of four that
+    span a row end (records run in serpentine order, so the step there is
+    one row) put the tiles holding them on the FP64 form (12 of 19) and the
+    others on the follower form."""
     lam, nx, ny = {"follower": (2.0, 64, 64), "follower_wide": (0.1, 64, 64), "fp64": (0.05, 64, 64),
-                   "both": (2.0, 16, 301)}[case]
+                   "both": (0.2, 16, 301)}[case]
     src = G.plane({"gmode": "xy", "lims_x": [-16, 16], "lims_y": [-16, 16], "gridsize": [nx, ny]})
     ff = G.plane({"gmode": "AoE", "lims_Az": [-30, 30], "lims_El": [-30, 30], "gridsize": [23, 19]})
     k = 2 * np.pi / lam
