@@ -218,13 +218,12 @@ __global__ __launch_bounds__(256) void k_ff_finish(const double *__restrict__ pa
                                                    double pref, OutC2 out) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_tgt) return;
-    double S[12];
+    double S[12];   // split sums; each split's twelve loads issue together
 #pragma unroll
-    for (int q = 0; q < 12; q++) {
-        double v = 0.0;
-        for (int s = 0; s < splits; s++) v += partial[((size_t)s * 12 + q) * n_tgt + t];
-        S[q] = v;
-    }
+    for (int q = 0; q < 12; q++) S[q] = partial[(size_t)q * n_tgt + t];
+    for (int s = 1; s < splits; s++)
+#pragma unroll
+        for (int q = 0; q < 12; q++) S[q] += partial[((size_t)s * 12 + q) * n_tgt + t];
     double st, ct, sp, cp;
     sincos(th[t], &st, &ct);
     sincos(ph[t], &sp, &cp);

@@ -229,13 +229,14 @@ __global__ __launch_bounds__(256) void k_near_finish(const double *__restrict__ 
     double S[12];
     // the JM sums write the H rows only
     constexpr int Q0 = (MODE == PYPO_MODE_JM) ? 6 : 0;
+    // split sums; each split's loads issue together
 #pragma unroll
-    for (int q = 0; q < 12; q++) {
-        double v = 0.0;
-        if (q >= Q0)
-            for (int s = 0; s < splits; s++) v += partial[((size_t)s * 12 + q) * n_tgt + t];
-        S[q] = pref * v;
-    }
+    for (int q = 0; q < 12; q++) S[q] = q >= Q0 ? partial[(size_t)q * n_tgt + t] : 0.0;
+    for (int s = 1; s < splits; s++)
+#pragma unroll
+        for (int q = Q0; q < 12; q++) S[q] += partial[((size_t)s * 12 + q) * n_tgt + t];
+#pragma unroll
+    for (int q = 0; q < 12; q++) S[q] *= pref;
     double Er[3], Ei[3], Hr[3], Hi[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
