@@ -10,12 +10,16 @@ One step = JM (feed -> dish currents, near-field kernel) + FF (dish -> far
 field), all buffers resident in HBM, mixed precision (FP32 integrand, FP64
 distance / phase).  value = (JM pairs + FF pairs) / seconds per step.
 
-Also reported: the all-FP64 GPU variant of the same step, the max error of
-both against the FP64 CPU checker on a strided subset of far-field rows, the
-FF kernel's own duration (HIP events on the launch stream) as the roofline
-figure, the ray tracer's rays/s on the Cassegrain configuration, and the
-CPU baseline timed on the host (reference CPU library where it was built,
-else this project's restatement) on the same subset.
+Also reported: the all-FP64 GPU variant of the same step (median of 3) and
+the same mixed step with the far field on the vector ALUs; the errors of
+both precisions against the all-FP64 CPU chain (reference CPU library where
+it was built, else this project's restatement) for the JM stage on every
+dish point, the FF stage and the whole chain on a strided subset of
+far-field rows; the FF kernel's own duration (HIP events on the launch
+stream) as the roofline figure, with the flop ceiling and the issue ceiling
+of its instruction stream; the ray tracer's rays/s on the Cassegrain
+configuration; and the CPU baseline (the chain's FF call on the row subset)
+timed on the host's cores.
 
 Secondary objects in the same line: "sweep" (the 256-instance batched sweep,
 autotuned, hipGraph replay) and "adjoint" (forward + backward of the
@@ -66,6 +70,24 @@ PAIR_TIME_FLOOR = (FF_FLOPS_MAC / PEAK_F16_MFMA + (FF_FLOPS_F32 - FF_FLOPS_MAC) 
                    + FF_FLOPS_F64 / PEAK_F64)
 PAIR_TIME_FLOOR_VECTOR = FF_FLOPS_F32 / PEAK_F32 + FF_FLOPS_F64 / PEAK_F64
 FF_KERNEL = "k_ff_mfma8"      # the far-field sum kernel of the default launch shape
+
+# Issue model of k_ff_mfma8's instruction stream: cycles one SIMD spends
+# issuing the vector instructions of 64 pairs (one pair per lane), from
+# per-instruction issue costs measured on gfx950 with one-instruction loops
+# at 4 waves per SIMD (FP32 FMA 3.6, FP64 FMA / v_fract_f64 / v_cvt_f32_f64
+# 4.8, v_cvt_pk_f16_f32 5.4, v_fma_mix_f32 5.5, v_sin_f32 / v_cos_f32 9.8
+# cycles per wave-instruction).  The 16x16x32 FP16 MFMAs (3 per 256 pairs)
+# run beside this on the matrix core and are not charged.  Unlike the flop
+# ceiling above, this one charges the sin/cos and the FP16 hi/lo operand
+# split, which the algorithmic flop count leaves out.
+ISSUE_CYCLES_PER_64_PAIRS = {
+    "leader_phase_fp64": (3 * 4.8 + 4.8 + 4.8) / 4,     # one FP64 phase per group of 4 sources
+    "follower_phase_fp32": 3 * 3 * 3.6 / 4,              # 3 FP32 FMAs for 3 of the 4 sources
+    "sin_cos": 2 * 9.8,
+    "fp16_hi_lo_split": 5.4 + 2 * 5.5 + 5.4,
+}
+SIMDS = 256 * 4               # MI355X: 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9              # MI355X peak engine clock
 
 
 def log(*a):
@@ -126,10 +148,6 @@ def measured_traffic(kernel):
             return float(json.load(f)[kernel]["bytes_per_launch"])
     except (OSError, KeyError, ValueError):
         return None
-
-
-def rel_err(a, b):
-    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
 def run_rt(torch, D):
@@ -260,35 +278,87 @@ def run_adjoint(torch, log):
                         "-> AoE 201x201 +-0.2deg, lambda=1mm; gradient w.r.t. 361201 normal displacements"}
 
 
-def cpu_baseline(dish, ff, cur_dish, sub_rows):
-    """FF on the strided row subset with the CPU checker library, timed."""
+def cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may run on
+    (its affinity mask), capped by OMP_NUM_THREADS where the host sets it
+    (the CPU share of one GPU on a shared GPU host).  Returns the count and
+    the figures it came from."""
+    total = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = total
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    n = aff
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return n, {"os_cpu_count": total, "affinity_cores": aff, "omp_num_threads": omp or None}
+
+
+def field_err(a, b):
+    """Max |a - b| relative to the peak |b|, per field (rows 0-2 and 3-5 of a
+    (6, n) complex array each have their own peak); the worse field."""
+    worst = 0.0
+    for g in (0, 3):
+        pk = np.max(np.abs(b[g:g + 3]))
+        if pk > 0:
+            worst = max(worst, float(np.max(np.abs(a[g:g + 3] - b[g:g + 3])) / pk))
+    return worst
+
+
+def cpu_chain(feed, dish, cur_feed, sub_rows, threads):
+    """The all-FP64 CPU chain with the reference CPU library (or this
+    project's restatement where the reference was not built): JM over the
+    whole dish, then FF on the strided far-field rows.  The FF call is the
+    timed CPU baseline; the same call on half the rows checks that the
+    sampled rate is linear in the rows."""
     from oracle import bind as OB
     from pypo_amd import geometry as G
     kind = "ref" if OB.available("ref") else "port"
     lib = OB.load(kind)
+    t = time.perf_counter()
+    jm, _, _ = OB.propagate(lib, "JM", feed, dish, cur_feed, K, threads)
+    jm_s = time.perf_counter() - t
     nrow = len(sub_rows)
     sub = G.plane({"gmode": "AoE", "lims_Az": [-FF_LIM, FF_LIM], "lims_El": [-FF_LIM, FF_LIM],
                    "gridsize": [nrow, N_FF]})
-    cores = min(os.cpu_count() or 1, int(os.environ.get("PYPO_BENCH_CPU_THREADS", "16")))
     t = time.perf_counter()
-    out, _, _ = OB.propagate(lib, "FF", dish, sub, cur_dish, K, cores)
+    out, _, _ = OB.propagate(lib, "FF", dish, sub, jm, K, threads)
     dt = time.perf_counter() - t
     vals = np.stack([c.ravel() for c in out.components()])
     pairs = N_DISH * N_DISH * nrow * N_FF
-    # linearity of the sampled rate: the same call on half the rows
     half = G.plane({"gmode": "AoE", "lims_Az": [-FF_LIM, FF_LIM], "lims_El": [-FF_LIM, FF_LIM],
                     "gridsize": [nrow // 2, N_FF]})
     t = time.perf_counter()
-    OB.propagate(lib, "FF", dish, half, cur_dish, K, cores)
+    OB.propagate(lib, "FF", dish, half, jm, K, threads)
     dt_half = time.perf_counter() - t
     rate_half = N_DISH * N_DISH * (nrow // 2) * N_FF / dt_half
-    return vals, {"value": pairs / dt, "unit": "pairs/s", "cores": cores,
-                  "kind": "reference" if kind == "ref" else "port",
-                  "sample": f"FF rows {sub_rows[0]}..{sub_rows[-1]} step {CPU_ROW_STRIDE} of the 501x501 "
-                            f"far field ({nrow * N_FF} targets x {N_DISH * N_DISH} sources = {pairs:.3e} "
-                            f"pairs), one native call incl. its grid generation, {dt:.1f} s",
-                  "linearity": {"rows": [nrow // 2, nrow], "pairs_per_s": [rate_half, pairs / dt],
-                                "ratio": round((pairs / dt) / rate_half, 3)}}
+    n_thr, counts = threads, cpu_threads()[1]
+    base = {"value": pairs / dt, "unit": "pairs/s", "cores": n_thr,
+            "kind": "reference" if kind == "ref" else "port",
+            "sample": f"FF rows {sub_rows[0]}..{sub_rows[-1]} step {CPU_ROW_STRIDE} of the 501x501 "
+                      f"far field ({nrow * N_FF} targets x {N_DISH * N_DISH} sources = {pairs:.3e} "
+                      f"pairs), one native call incl. its grid generation, {dt:.1f} s",
+            "threads_from": "affinity mask, capped by OMP_NUM_THREADS", **counts,
+            "compile_flags": "g++ -O3 -pthread -fPIC (CMakeLists.txt:11 release flags)",
+            "jm_full_dish_s": round(jm_s, 2),
+            "linearity": {"rows": [nrow // 2, nrow], "pairs_per_s": [rate_half, pairs / dt],
+                          "ratio": round((pairs / dt) / rate_half, 3)}}
+    jm_vals = np.stack([c.ravel() for c in jm.components()])
+    return vals, jm_vals, jm, base
+
+
+def median_time(fn, reps):
+    """Median and spread (ms) of `reps` synchronised calls of fn()."""
+    import torch
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), [round(min(ts) * 1e3, 2), round(max(ts) * 1e3, 2)]
 
 
 def main():
@@ -389,44 +459,61 @@ def main():
     peak_vector = FF_FLOPS / PAIR_TIME_FLOOR_VECTOR / 1e12
 
     res = {}
-    ff64 = None
+    issue_cycles = sum(ISSUE_CYCLES_PER_64_PAIRS.values())
+    issue_ceiling = SIMDS * CLOCK_HZ * 64 / issue_cycles
+    ff64 = jm64_host = None
     if not args.no_fp64:
         step("fp64")
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        step("fp64")
-        torch.cuda.synchronize()
-        dt64 = time.perf_counter() - t
+        dt64, spread64 = median_time(lambda: step("fp64"), 3)
         ff64 = D.tensor_to_complex6(out_ff)
-        # restore the mixed currents/field for the error check below
+        jm64_host = D.tensor_to_complex6(cur_dish)
         res["fp64_ms_per_step"] = round(dt64 * 1e3, 2)
+        res["fp64_ms_per_step_range"] = spread64
+        res["fp64_timing"] = "median of 3 synchronised steps"
         res["fp64_pairs_per_s"] = (pairs_jm + pairs_ff) / dt64
         res["mixed_speedup_vs_fp64"] = round(dt64 / dt, 2)
-        res["mixed_vs_fp64_gpu_max_err_rel_peak"] = rel_err(ff_mixed, ff64)
-        log(f"fp64 step {dt64 * 1e3:.1f} ms, mixed vs fp64 err {res['mixed_vs_fp64_gpu_max_err_rel_peak']:.2e}")
+        res["mixed_vs_fp64_gpu_max_err_rel_peak"] = field_err(ff_mixed, ff64)
+        log(f"fp64 step {dt64 * 1e3:.1f} ms (median of 3, range {spread64}), "
+            f"mixed vs fp64 err {res['mixed_vs_fp64_gpu_max_err_rel_peak']:.2e}")
+        # the same mixed step with the far field on the vector ALUs (FP32 FMAs)
+        B.set_ff_kernel("vector")
+        try:
+            step("mixed")
+            dtv, spreadv = median_time(lambda: step("mixed"), 3)
+        finally:
+            B.set_ff_kernel("matrix")
+        res["fp32_vector_ms_per_step"] = round(dtv * 1e3, 2)
+        res["fp32_vector_ms_per_step_range"] = spreadv
+        res["fp32_vector_speedup_vs_fp64"] = round(dt64 / dtv, 2)
+        log(f"vector-ALU far field step {dtv * 1e3:.1f} ms ({dt64 / dtv:.2f}x the fp64 step)")
 
     cpu = None
     if not args.no_cpu:
         from pypo_amd import types as T
         rows = list(range(0, N_FF, CPU_ROW_STRIDE))
-        # FP64 checker on the dish currents that the mixed JM step produced
-        cur_obj = T.currents(*[c.reshape(N_DISH, N_DISH) for c in cur_host])
-        log(f"cpu baseline on {len(rows)} far-field rows ...")
-        vals, cpu = cpu_baseline(dish, ff, cur_obj, rows)
+        threads, _ = cpu_threads()
+        log(f"cpu chain: JM over the whole dish, then FF on {len(rows)} far-field rows, {threads} threads ...")
+        vals, jm_cpu, jm_obj, cpu = cpu_chain(feed, dish, feed_currents(w0), rows, threads)
         idx = (np.asarray(rows)[:, None] * N_FF + np.arange(N_FF)[None, :]).ravel()
-        # the checker ran on the mixed-step currents: compare the FF stage
-        D.propagate("FF", g_dish, g_ff, D.currents_tensor(cur_obj), dish, K, precision="mixed",
-                    out=out_ff)
-        ffm = D.tensor_to_complex6(out_ff)[:, idx]
-        res["ff_mixed_max_err_rel_peak_vs_cpu_fp64"] = rel_err(ffm, vals)
-        if not args.no_fp64:
-            D.propagate("FF", g_dish, g_ff, D.currents_tensor(cur_obj), dish, K, precision="fp64",
-                        out=out_ff)
-            ff64s = D.tensor_to_complex6(out_ff)[:, idx]
-            res["ff_fp64_max_err_rel_peak_vs_cpu_fp64"] = rel_err(ff64s, vals)
+        # JM stage: the induced J on every dish point (M is zero for the PEC currents)
+        res["jm_mixed_max_err_rel_peak_vs_cpu_fp64"] = field_err(cur_host, jm_cpu)
+        # whole chain: the timed step's field (mixed JM -> mixed FF) against the FP64 CPU chain
+        res["chain_mixed_max_err_rel_peak_vs_cpu_fp64"] = field_err(ff_mixed[:, idx], vals)
+        # FF stage alone: the GPU far field of the CPU's FP64 currents
+        cur_cpu = D.currents_tensor(jm_obj)
+        D.propagate("FF", g_dish, g_ff, cur_cpu, dish, K, precision="mixed", out=out_ff)
+        res["ff_mixed_max_err_rel_peak_vs_cpu_fp64"] = field_err(D.tensor_to_complex6(out_ff)[:, idx], vals)
+        if ff64 is not None:
+            res["jm_fp64_max_err_rel_peak_vs_cpu_fp64"] = field_err(jm64_host, jm_cpu)
+            res["chain_fp64_max_err_rel_peak_vs_cpu_fp64"] = field_err(ff64[:, idx], vals)
+            D.propagate("FF", g_dish, g_ff, cur_cpu, dish, K, precision="fp64", out=out_ff)
+            res["ff_fp64_max_err_rel_peak_vs_cpu_fp64"] = field_err(D.tensor_to_complex6(out_ff)[:, idx], vals)
         torch.cuda.synchronize()
+        res["err_sample"] = (f"JM: all {n_dish} dish points; FF and chain: far-field rows {rows[0]}.."
+                             f"{rows[-1]} step {CPU_ROW_STRIDE} ({idx.size} targets, full sums); "
+                             f"max |error| / peak per field (J; E and H)")
         res["speedup_vs_cpu_ff"] = round(ff_pairs_s / cpu["value"], 1)
-        log(f"cpu {cpu['value']:.3e} pairs/s; errors {res}")
+        log(f"cpu {cpu['value']:.3e} pairs/s; {res}")
 
     if not args.no_rt:
         rays_s, rt_ms, nrays, rt_kms = run_rt(torch, D)
@@ -465,7 +552,14 @@ def main():
                      "work_per_pair": f"{FF_FLOPS_F32} FP32 ({FF_FLOPS_MAC} of them multiply-adds, run on "
                                       f"the FP16 matrix cores) + {FF_FLOPS_F64} FP64 flops + sincos",
                      "peak_all_vector": round(peak_vector, 1),
-                     "frac_all_vector": round(achieved / peak_vector, 4)},
+                     "frac_all_vector": round(achieved / peak_vector, 4),
+                     "issue_ceiling": {"pairs_per_s": issue_ceiling,
+                                       "frac": round(ff_pairs_s / issue_ceiling, 4),
+                                       "cycles_per_64_pairs": {k: round(v, 2)
+                                                               for k, v in ISSUE_CYCLES_PER_64_PAIRS.items()},
+                                       "basis": "vector issue of this kernel's instruction stream at "
+                                                "measured gfx950 per-instruction costs, 1024 SIMDs x 2.4 GHz; "
+                                                "charges sin/cos and the fp16 hi/lo split"}},
         "cpu_baseline": cpu,
     }
     line.update(res)

@@ -238,7 +238,12 @@ __global__ __launch_bounds__(256) void k_nn_query(const double *__restrict__ qx,
         }
         if (best < bound * (1.0 - 1e-12)) break;   // bound = inf: every cell searched
     }
-    // best_i >= 0 whenever there is at least one point (the host guarantees it)
+    // best_i < 0 only when no point lies at a finite distance (NaN
+    // coordinates in the query or in every point): the value is NaN
+    if (best_i < 0) {
+        for (int v = 0; v < nvals; v++) vout.p[v][t] = NAN;
+        return;
+    }
     for (int v = 0; v < nvals; v++) vout.p[v][t] = vin.p[v][best_i];
 }
 

@@ -14,8 +14,8 @@ Also mirrored: beam and frame creation (createTubeFrame 1243-1266,
 createGRTFrame 1268-1302, createGaussian 1373-1404, createGaussianBeam
 1406-1445, createScalarGaussian 1447-1472, createPointSource /
 createUniformSource and their scalar twins 2146-2311), calcRayLen
-(1304-1372), runHybridPropagation (1570-1622) and interpFrame (1624-1692,
-nearest-neighbour method); the frame and beam figures of merit
+(1304-1372), runHybridPropagation (1570-1622) and interpFrame (1624-1692;
+nearest neighbour on the GPU, scipy's other methods on the host); the frame and beam figures of merit
 (calcRTcenter / calcRTtilt / calcSpotRMS 1694-1746, calcSpillover / calcTaper
 / calcXpol 1748-1826) as GPU reductions; findRTfocus (2632-2700) over GPU
 traces; and the grid-convergence loops autoConverge / convergeOnTarget
@@ -49,6 +49,23 @@ class HybridPropError(Exception):
 
 
 FIELD_COMPONENTS = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
+
+
+def _device_alias(d, what):
+    """The runPODict / runRTDict "device" key (Checks.py:1566-1591, 1023-1047).
+    The reference runs "CPU" on its threaded C++ library and falls back to it,
+    with a warning, for an unknown name or a missing GPU library.  This
+    library has one implementation, on the GPU: "CPU" is accepted as an alias
+    so that scripts written for the reference run unchanged, an unknown name
+    is warned about as there, and the key is then set to "GPU".  nThreads is
+    accepted and ignored (the launch shape is the library's)."""
+    dev = d.setdefault("device", "CPU")
+    if dev not in ("CPU", "GPU", "HIP"):
+        log.warning(f"Device {dev} unknown. Running {what} on the GPU.")
+    elif dev == "CPU":
+        log.info(f"device CPU: {what} runs on the GPU (this library has no CPU path).")
+    d["device"] = "GPU"
+    d.setdefault("nThreads", 256)
 
 
 def _comp_index(comp):
@@ -355,10 +372,7 @@ class System:
         d.setdefault("precision", "mixed")
         if d["precision"] not in B.PRECISIONS:
             err.append(f"{d['precision']} is not a valid precision (mixed, fp64).")
-        dev = d.setdefault("device", "GPU")
-        if dev not in ("GPU", "HIP"):
-            err.append(f"device {dev}: this library runs PO on the GPU only (GPU or HIP).")
-        d.setdefault("nThreads", 256)
+        _device_alias(d, "PO")
         if err:
             for e in err:
                 log.error(e)
@@ -434,10 +448,7 @@ class System:
         d["tol"] = tol
         t0 = d.get("t0", 1)
         d["t0"] = t0 if isinstance(t0, (int, float)) else 1
-        dev = d.setdefault("device", "GPU")
-        if dev not in ("GPU", "HIP"):
-            err.append(f"device {dev}: this library traces rays on the GPU only (GPU or HIP).")
-        d.setdefault("nThreads", 256)
+        _device_alias(d, "ray tracing")
         if err:
             for e in err:
                 log.error(e)
@@ -595,18 +606,35 @@ class System:
 
     def interpFrame(self, name_fr_in, name_field, name_target, name_out, comp=None, method="nearest"):
         """Interpolate a field carried by a frame onto surface `name_target`'s grid
-        (System.interpFrame, System.py:1624-1692) -- nearest neighbour, on the GPU."""
+        (System.interpFrame, System.py:1624-1692).  "nearest" runs on the GPU
+        (a nearest-neighbour search with scipy griddata's result); any other
+        method is handed, as in the reference, to scipy.interpolate.griddata
+        on the host with the frame's points and the grid generated on the
+        device (griddata supports "linear" on these 3-D points and raises for
+        "cubic", as it does for the reference)."""
         from . import device as D
-        if method != "nearest":
-            raise ValueError("interpFrame runs the nearest-neighbour method on the GPU only")
         if name_fr_in not in self.frames:
             raise RunRTError(f"Frame {name_fr_in} not in system.")
         field = self.fields[name_field]
         grid = D.generate_grid(self._surface(name_target))
-        out = D.interp_nearest(D.frame_tensor(self.frames[name_fr_in]), D.currents_tensor(field), grid)
         shape = tuple(int(v) for v in self.system[name_target]["gridsize"])
-        comps = [c.reshape(shape) for c in D.tensor_to_complex6(out)]
+        fr = self.frames[name_fr_in]
         ci = _comp_index(comp)
+        if method == "nearest":
+            out = D.interp_nearest(D.frame_tensor(fr), D.currents_tensor(field), grid)
+            comps = [c.reshape(shape) for c in D.tensor_to_complex6(out)]
+        else:
+            from scipy.interpolate import griddata
+            g = grid.cpu().numpy()
+            points = (np.ravel(fr.x), np.ravel(fr.y), np.ravel(fr.z))
+            comps = [None] * 6
+            for i, c in enumerate(field.components()):
+                if ci is not None and i != ci:
+                    continue
+                c = np.ravel(c)
+                re = griddata(points, np.real(c), (g[0], g[1], g[2]), method=method)
+                im = griddata(points, np.imag(c), (g[0], g[1], g[2]), method=method)
+                comps[i] = re.reshape(shape) + 1j * im.reshape(shape)
         if ci is None:
             res = T.fields(*comps)
             ret = res

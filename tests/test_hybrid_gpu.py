@@ -117,3 +117,19 @@ def test_system_hybrid_matches_checker(hip, port):
     assert group_err(got, ref) <= 1e-10
     assert s.assoc["focal"] == ["foc", "hit"]
     assert np.max(np.abs(s.frames["hit"].x - ref_fr.x)) <= 1e-9
+
+
+def test_hybrid_step_closed_form(hip):
+    """The GPU hybrid step on geometry worked out by hand: rays from the
+    origin sampled on z = 100, traced onto the plane z = 300, carry
+    exp(j k (z2 - z1) |p| / z1) sqrt(z1 / (2 z2 - z1)) (test_hybrid_oracle.py
+    pins the checker on the same case)."""
+    from pypo_amd import device as D
+    from test_hybrid_oracle import K, Z1, Z2, _case, closed_form
+    fr, target, comps, rho = _case(seed=5)
+    fr_t, f_t = D.hybrid(target, D.frame_tensor(fr), D.currents_tensor(T.fields(*comps)), K,
+                         start=np.zeros(3), tol=1e-10, t0=150.0)
+    got = D.tensor_to_complex6(f_t) / np.stack(comps)
+    assert np.max(np.abs(got - closed_form(rho)[None, :])) <= 1e-9
+    np.testing.assert_allclose(D.tensor_to_frame(fr_t).z, Z2, rtol=0, atol=1e-10)
+    assert Z1 < Z2

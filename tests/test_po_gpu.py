@@ -286,6 +286,45 @@ def test_large_far_field_mixed_vs_fp64_and_transverse(hip, port):
     assert e64 <= 1e-10 and em <= 1e-5
 
 
+def test_large_jm_mixed_and_chain_against_cpu(hip, port):
+    """The large configuration's mixed-precision step end to end against an
+    all-FP64 CPU chain: the JM stage (33 x 33 feed -> 1001 x 1001 dish,
+    1.09e9 pairs) on every dish point, rim included, and the mixed JM -> mixed
+    FF chain on far-field rows 0, 100, ..., 500 (3006 targets x 1e6 sources)
+    against the CPU's FP64 JM -> FP64 FF.  Bar: 1e-5 of each field's peak."""
+    import torch
+    import bench
+    from helpers import grid_rows, current_rows
+    from pypo_amd import device as D
+    w0 = bench.feed_waist()
+    feed, dish, ff = bench.surfaces()
+    cur_feed = bench.feed_currents(w0)
+    g_feed, g_dish, g_ff = D.generate_grid(feed), D.generate_grid(dish), D.generate_grid(ff)
+    jm = D.propagate("JM", g_feed, g_dish, D.currents_tensor(cur_feed), feed, bench.K, precision="mixed")
+    out = D.propagate("FF", g_dish, g_ff, jm, dish, bench.K, precision="mixed")
+    torch.cuda.synchronize()
+    J = D.tensor_to_complex6(jm)
+    # FP64 CPU chain: JM over the whole dish, then FF on the strided rows
+    jm_cpu = OB.propagate_on_grids(port, "JM", grid_rows(port, feed), (feed["gmode"], bench.N_FEED, bench.N_FEED),
+                                   grid_rows(port, dish), current_rows(cur_feed), bench.K)
+    Jc = np.stack([jm_cpu[0] + 1j * jm_cpu[3], jm_cpu[1] + 1j * jm_cpu[4], jm_cpu[2] + 1j * jm_cpu[5],
+                   jm_cpu[6] + 1j * jm_cpu[9], jm_cpu[7] + 1j * jm_cpu[10], jm_cpu[8] + 1j * jm_cpu[11]])
+    assert np.all(Jc[3:] == 0) and np.all(J[3:] == 0)   # PEC currents: no M
+    e_jm = group_err(J[:3], Jc[:3])
+    rim = np.arange((bench.N_DISH - 1) * bench.N_DISH, bench.N_DISH ** 2)
+    e_rim = float(np.max(np.abs(J[:3, rim] - Jc[:3, rim])) / np.max(np.abs(Jc[:3])))
+    e_str = group_err(J[:3, ::97], Jc[:3, ::97])
+    print(f"large JM mixed vs CPU FP64: all {e_jm:.2e}, every 97th {e_str:.2e}, rim {e_rim:.2e}")
+    assert e_jm <= 1e-5
+    rows = np.arange(0, bench.N_FF, bench.CPU_ROW_STRIDE)
+    idx = (rows[:, None] * bench.N_FF + np.arange(bench.N_FF)[None, :]).ravel()
+    cur_cpu = T.currents(*[c.reshape(bench.N_DISH, bench.N_DISH) for c in Jc])
+    vals, _ = OB.farfield_subset(port, dish, ff, cur_cpu, bench.K, idx)
+    e_chain = group_err(D.tensor_to_complex6(out)[:, idx], vals)
+    print(f"large JM -> FF mixed chain vs CPU FP64 chain on {idx.size} targets: {e_chain:.2e}")
+    assert e_chain <= 1e-5
+
+
 def test_bad_arguments_raise(hip):
     src = G.plane({"gmode": "xy", "lims_x": [-1, 1], "lims_y": [-1, 1], "gridsize": [3, 3]})
     z = np.zeros((3, 3), complex)

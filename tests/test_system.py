@@ -44,10 +44,24 @@ def test_invalid_runPODict_raises():
     with pytest.raises(RunPOError):
         s.runPO({"t_name": "dish", "mode": "EHP", "s_current": "feedJM", "name_EH": "a"})
     with pytest.raises(RunPOError):
-        s.runPO({"t_name": "dish", "mode": "JM", "s_current": "feedJM", "name_JM": "a", "device": "CPU"})
-    with pytest.raises(RunPOError):
         s.runPO({"t_name": "dish", "mode": "JM", "s_current": "feedJM", "name_JM": "a",
                  "precision": "half"})
+
+
+def test_device_key_is_an_alias():
+    """device "CPU" (the reference's default) and unknown names are accepted
+    and run on the GPU, as the reference accepts them (Checks.py:1566-1591)."""
+    s = make_system()
+    for dev in ("CPU", "GPU", "HIP", "TPU", None):
+        d = {"t_name": "dish", "mode": "JM", "s_current": "feedJM", "name_JM": "a"}
+        if dev is not None:
+            d["device"] = dev
+        s._check_runPODict(d)
+        assert d["device"] == "GPU" and d["nThreads"] == 256
+    s.addFrame("f", T.frame(1, [0.0], [0.0], [50.0], [0.0], [0.0], [-1.0]))
+    d = {"fr_in": "f", "t_name": "dish", "fr_out": "o", "device": "CPU"}
+    s._check_runRTDict(d)
+    assert d["device"] == "GPU"
 
 
 def test_invalid_runRTDict_raises():
