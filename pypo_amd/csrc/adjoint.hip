@@ -207,12 +207,13 @@ __device__ __forceinline__ void build_rev_b(const TgtRec32 &r, int sl, float inv
         for (int q = 0; q < 3; q++) {
             const float re = r.a[2 * q] * w[v] * inv, im = r.a[2 * q + 1] * w[v] * inv;
             unsigned h, l;
+            const int s0 = mfma::b_slot(ks, 16 * g + c0 + 2 * q), s1 = mfma::b_slot(ks, 16 * g + c0 + 2 * q + 1);
             mfma::split_pair_fast(re, -im, h, l);
-            reinterpret_cast<unsigned *>(&bh[ks][16 * g + c0 + 2 * q])[jj] = h;
-            reinterpret_cast<unsigned *>(&bl[ks][16 * g + c0 + 2 * q])[jj] = l;
+            reinterpret_cast<unsigned *>(&bh[ks][s0])[jj] = h;
+            reinterpret_cast<unsigned *>(&bl[ks][s0])[jj] = l;
             mfma::split_pair_fast(im, re, h, l);
-            reinterpret_cast<unsigned *>(&bh[ks][16 * g + c0 + 2 * q + 1])[jj] = h;
-            reinterpret_cast<unsigned *>(&bl[ks][16 * g + c0 + 2 * q + 1])[jj] = l;
+            reinterpret_cast<unsigned *>(&bh[ks][s1])[jj] = h;
+            reinterpret_cast<unsigned *>(&bl[ks][s1])[jj] = l;
         }
     }
 }
@@ -238,7 +239,8 @@ __device__ __forceinline__ void rev_tile_mfma(const RevTile &tt, const mfma::hal
     auto tgt_of = [](int j) { return FAST ? (j == 0 ? 1 : (j == 1 ? 0 : j)) : j; };
 #pragma unroll 1
     for (int ks = 0; ks < mfma::KSTEPS; ks++) {
-        const half8 h0 = b0h[ks][lane], l0 = b0l[ks][lane], h1 = b1h[ks][lane], l1 = b1l[ks][lane];
+        const int slot = mfma::b_slot(ks, lane);
+        const half8 h0 = b0h[ks][slot], l0 = b0l[ks][slot], h1 = b1h[ks][slot], l1 = b1l[ks][slot];
         const int t0 = ks * 16 + kb * 4;
         double ux[4], uy[4], uz[4];
         float dx[4], dy[4], dz[4], rx[4], ry[4], rz[4];
@@ -302,7 +304,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int row = lane & 15, kb = lane >> 4;
     const int s0 = blockIdx.x * (64 * MB) + wave * (16 * MB);
     {
-        const int ks = tid >> 4, l = 16 * ((tid >> 2) & 3) + 12 + (tid & 3);
+        const int ks = tid >> 4, l = mfma::b_slot(ks, 16 * ((tid >> 2) & 3) + 12 + (tid & 3));
         b0h[ks][l] = half8{}; b0l[ks][l] = half8{}; b1h[ks][l] = half8{}; b1l[ks][l] = half8{};
     }
     double px[MB], py[MB], pz[MB];

@@ -50,7 +50,8 @@
 // one FP32 source record from HBM (prefetched during the previous tile),
 // writes its coordinates to LDS (one array per coordinate: the four lane
 // groups of a wave then read distinct banks) and its B entries (16 K-steps
-// x 64 lanes x 16 B, hi and lo in separate arrays: 32 KB of LDS).  Lane layout of the
+// x 64 lanes x 16 B, hi and lo in separate arrays: 32 KB of LDS; lane slots
+// swizzled within each K-step, b_slot).  Lane layout of the
 // 16x16x32 f16 MFMA (checked on gfx950): lane l holds A[l % 16][8 (l / 16) + j],
 // B[8 (l / 16) + j][l % 16] (j = 0..7) and D[4 (l / 16) + i][l % 16] (i = 0..3).
 
@@ -95,6 +96,14 @@ __device__ __forceinline__ void split_pair_fast(float c, float s, unsigned &hi, 
     lo = __builtin_bit_cast(unsigned, half2v{(_Float16)dc, (_Float16)ds});
 }
 
+// Slot of lane L's B operand in a K-step row of 64 half8 (16-byte) slots:
+// the low four bits of L are XORed with (ks mod 4, L / 16).  build_b's
+// stores of one column come from the 16 lanes of a wave that differ in
+// (ks mod 4, g): unswizzled they all hit one bank group (a 16-way
+// conflict); swizzled they hit 16 different ones.  A bijection within the
+// row, so a wave's 64 reads of one K-step stay one contiguous 1 KB block.
+__device__ __forceinline__ int b_slot(int ks, int L) { return L ^ (((ks & 3) << 2) | (L >> 4)); }
+
 // This lane's source record (tile slot sl) -> its B entries of every column.
 // Source slot sl sits in K-step sl / 16, lane group g = (sl % 16) / 4, K rows
 // 2 (sl % 4) (cos) and 2 (sl % 4) + 1 (sin) of the lanes 16 g + q.
@@ -106,14 +115,15 @@ __device__ __forceinline__ void build_b(const SrcRec32 &r, int sl, float inv, ha
         const float re = (a < 3 ? r.cj[2 * a] : r.cm[2 * (a - 3)]) * inv;
         const float im = (a < 3 ? r.cj[2 * a + 1] : r.cm[2 * (a - 3) + 1]) * inv;
         unsigned h, l;
+        const int c0 = b_slot(ks, 16 * g + 2 * a), c1 = b_slot(ks, 16 * g + 2 * a + 1);
         // real column 2a: cos row Re, sin row -Im
         split_pair_fast(re, -im, h, l);
-        reinterpret_cast<unsigned *>(&bhi[ks][16 * g + 2 * a])[jj] = h;
-        reinterpret_cast<unsigned *>(&blo[ks][16 * g + 2 * a])[jj] = l;
+        reinterpret_cast<unsigned *>(&bhi[ks][c0])[jj] = h;
+        reinterpret_cast<unsigned *>(&blo[ks][c0])[jj] = l;
         // imaginary column 2a+1: cos row Im, sin row Re
         split_pair_fast(im, re, h, l);
-        reinterpret_cast<unsigned *>(&bhi[ks][16 * g + 2 * a + 1])[jj] = h;
-        reinterpret_cast<unsigned *>(&blo[ks][16 * g + 2 * a + 1])[jj] = l;
+        reinterpret_cast<unsigned *>(&bhi[ks][c1])[jj] = h;
+        reinterpret_cast<unsigned *>(&blo[ks][c1])[jj] = l;
     }
 }
 
@@ -211,7 +221,8 @@ __device__ __forceinline__ void ff_tile_mfma(const TileXYZ &tx, const half8 (*bh
     const int kb = lane >> 4;
 #pragma unroll 1
     for (int ks = 0; ks < KSTEPS; ks++) {
-        const half8 b_hi = bhi[ks][lane], b_lo = blo[ks][lane];
+        const int slot = b_slot(ks, lane);
+        const half8 b_hi = bhi[ks][slot], b_lo = blo[ks][slot];
         ff_kstep<MB, MASK, FAST>(tx, b_hi, b_lo, ks * 16 + kb * 4, qx, qy, qz, tq, lt0, acc);
     }
 }
@@ -234,7 +245,7 @@ __device__ __forceinline__ void ff_mfma_body(const SrcRec32 *__restrict__ src, c
 
     // the columns 12..15 of B stay zero
     {
-        const int ks = tid >> 4, l = 16 * ((tid >> 2) & 3) + 12 + (tid & 3);
+        const int ks = tid >> 4, l = b_slot(ks, 16 * ((tid >> 2) & 3) + 12 + (tid & 3));
         bhi[ks][l] = half8{};
         blo[ks][l] = half8{};
     }
