@@ -219,6 +219,30 @@ int pypo_hip_propagate_dev(int mode, void *res, const reflcontainer *cs,
                            int gmode, int ncx, int ncy, double k, double epsilon,
                            double t_direction, int precision, void *stream);
 
+/* Size of the device scratch of pypo_hip_propagate_noalloc: 128 MiB per
+ * device, held from the library module's load until the process ends. */
+#define PYPO_HIP_NOALLOC_SCRATCH_BYTES (128u * 1024u * 1024u)
+
+/* pypo_hip_propagate_dev without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate (serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, results and ordering as
+ * pypo_hip_propagate_dev; the packed sources and split partial sums live in
+ * the library's static device scratch (PYPO_HIP_NOALLOC_SCRATCH_BYTES per
+ * device) instead of its grow-only pool, with as many source splits as fit
+ * (bit-identical to pypo_hip_propagate_dev whenever its split plan fits).
+ * A problem whose packed sources plus one split of partials exceed the
+ * scratch is rejected with ERR_ARG (1001) before anything runs.  Every call
+ * on a device shares its scratch, which keeps the last call's contents: the
+ * library orders its calls with one completion event, so calls from several
+ * threads or streams run one after another; work replayed outside the
+ * library's ordering (e.g. from a captured graph) would race on it.  Calls
+ * no hipMalloc/hipFree; the first library call of a process creates the
+ * library's stream and completion event. */
+int pypo_hip_propagate_noalloc(int mode, void *res, const reflcontainer *cs,
+                               const reflcontainer *ct, const void *currents,
+                               int gmode, int ncx, int ncy, double k, double epsilon,
+                               double t_direction, int precision, void *stream);
+
 /* Device-resident grid generation into device arrays of container. */
 int pypo_hip_generate_grid_dev(reflparams refl, reflcontainer *container, void *stream);
 

@@ -90,6 +90,8 @@ def loadHIPlib():
                                                P(PS.reflcontainer), ctypes.c_void_p, i, i, i, d, d, d,
                                                i, ctypes.c_void_p]
         lib.pypo_hip_propagate_dev.restype = i
+        lib.pypo_hip_propagate_noalloc.argtypes = lib.pypo_hip_propagate_dev.argtypes
+        lib.pypo_hip_propagate_noalloc.restype = i
         lib.pypo_hip_generate_grid_dev.argtypes = [PS.reflparams, P(PS.reflcontainer), ctypes.c_void_p]
         lib.pypo_hip_generate_grid_dev.restype = i
         lib.pypo_hip_trace_dev.argtypes = [P(PS.reflparams), i, P(PS.cframe), d, d, ctypes.c_void_p]

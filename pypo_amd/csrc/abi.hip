@@ -28,6 +28,17 @@ void set_error(int code, const std::string &msg) {
     }
 }
 
+}  // namespace pypo
+
+// Device scratch of pypo_hip_propagate_noalloc: the packed source records,
+// their tile metadata and the split partial sums of one propagation.  Each
+// device the library runs on holds its own copy from the module's load until
+// the process ends.  In host code `scratch` names only a stand-in for the
+// device symbol: its device address comes from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[PYPO_HIP_NOALLOC_SCRATCH_BYTES];
+
+namespace pypo {
+
 // ---- library state ------------------------------------------------------------
 namespace {
 
@@ -742,6 +753,79 @@ int pypo_hip_propagate_dev(int mode, void *res, const reflcontainer *cs, const r
     if (err) return err;
     err = launch_po(mode, precision, packed, gs, grid_ptrs(*ct), gt, k, epsilon, t_direction, res, ws,
                     wsb, st, g_timing);
+    if (err) return err;
+    return end_call(st);
+}
+
+// ---- allocation-free propagation ---------------------------------------------------
+
+namespace {
+constexpr int kMaxDevices = 64;
+void *g_scratch[kMaxDevices] = {};   // device address of each device's scratch
+
+// This device's scratch address, fetched on the first call on the device
+// with that device current.
+int scratch_of_current_device(void **out) {
+    int dev = -1;
+    PYPO_HIP_TRY(hipGetDevice(&dev));
+    PYPO_REQUIRE(dev >= 0 && dev < kMaxDevices, ERR_ARG, "no-alloc propagation: device index out of range");
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        PYPO_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PYPO_REQUIRE(p != nullptr, ERR_ARG, "no-alloc propagation: scratch symbol has no device address");
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+}  // namespace
+
+int pypo_hip_propagate_noalloc(int mode, void *res, const reflcontainer *cs, const reflcontainer *ct,
+                               const void *currents, int gmode, int ncx, int ncy, double k,
+                               double epsilon, double t_direction, int precision, void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    PYPO_REQUIRE(mode >= PYPO_MODE_JM && mode <= PYPO_MODE_SCALAR, ERR_ARG, "unknown PO mode");
+    PYPO_REQUIRE(precision == PYPO_PREC_MIXED || precision == PYPO_PREC_FP64, ERR_ARG,
+                 "unknown precision");
+    PYPO_REQUIRE(res && currents && cs && ct, ERR_ARG, "NULL argument");
+    if ((err = check_container(cs, "source")) || (err = check_container(ct, "target"))) return err;
+    const int gs = cs->size, gt = ct->size;
+    PYPO_REQUIRE(gs > 0 && gt > 0 && (long long)ncx * ncy == gs, ERR_ARG,
+                 "ncx*ncy must equal the source size and both grids must be non-empty");
+    // scratch = [packed records + tile metadata | split partials]; the
+    // partials take at most as many source splits as fit behind the records
+    const size_t cap = (size_t)PYPO_HIP_NOALLOC_SCRATCH_BYTES;
+    const size_t pk = (packed_bytes(mode, precision, gs) + 255) / 256 * 256;
+    const size_t per_split = (size_t)(mode == PYPO_MODE_SCALAR ? 2 : 12) * gt * sizeof(double);
+    if (pk + per_split > cap) {
+        set_error(ERR_ARG, "pypo_hip_propagate_noalloc: " + std::to_string(gs) + " sources -> " +
+                               std::to_string(gt) + " targets need " + std::to_string(pk + per_split) +
+                               " bytes of scratch, more than its " + std::to_string(cap) +
+                               " bytes; use pypo_hip_propagate_dev");
+        return ERR_ARG;
+    }
+    void *base = nullptr;
+    if ((err = scratch_of_current_device(&base))) return err;
+    const long long fit = (long long)((cap - pk) / per_split);
+    const SumPlan plan = plan_cap_splits(plan_sum(mode, precision, gs, gt), gs, (int)std::min(fit, 1LL << 30));
+    double *partial = (double *)((char *)base + pk);
+    const size_t wsb = cap - pk;
+    PYPO_REQUIRE(partial_bytes(mode, plan, gt) <= wsb, ERR_ARG, "internal: split plan exceeds the scratch");
+    hipStream_t st = (hipStream_t)stream;   // NULL: the default stream
+    if ((err = begin_call(st))) return err;
+    GridPtrs gsrc = grid_ptrs(*cs);
+    if (mode == PYPO_MODE_SCALAR) {
+        const arrC1 *f = (const arrC1 *)currents;
+        err = launch_pack_scalar(precision, gsrc, gs, f->x, f->y, gmode, ncx, ncy, base, st);
+    } else {
+        err = launch_pack_sources(precision, gsrc, gs, bundle12(*(const c2Bundle *)currents), gmode, ncx,
+                                  ncy, base, st);
+    }
+    if (err) return err;
+    err = launch_po_plan(plan, mode, precision, base, gs, grid_ptrs(*ct), gt, k, epsilon, t_direction, res,
+                         partial, wsb, st, false);
     if (err) return err;
     return end_call(st);
 }

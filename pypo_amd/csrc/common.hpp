@@ -234,6 +234,8 @@ int set_sum_shape(int tpl, int want_wgs);   // 0, 0: built-in choice
 int set_ff_kernel(int kind);                // PYPO_FF_MATRIX or PYPO_FF_VECTOR
 int ff_kernel();
 size_t partial_bytes(int mode, const SumPlan &p, int n_tgt);
+// p with at most max_splits source splits (whole tiles per split).
+SumPlan plan_cap_splits(const SumPlan &p, int n_src, int max_splits);
 // Compute units of the current device (a launch-shape hint; 256 if the
 // query fails).
 int device_cus();
@@ -248,6 +250,10 @@ int tail_splits(long long blocks, long long tiles, int lo, int hi, long long slo
 int launch_po(int mode, int precision, const void *packed, int n_src, const GridPtrs &ct,
               int n_tgt, double k, double epsilon, double t_dir, void *res, double *partial_ws,
               size_t ws_bytes, hipStream_t st, bool time_it);
+// The same with a given launch plan (plan_sum's, or one derived from it).
+int launch_po_plan(const SumPlan &p, int mode, int precision, const void *packed, int n_src,
+                   const GridPtrs &ct, int n_tgt, double k, double epsilon, double t_dir, void *res,
+                   double *partial_ws, size_t ws_bytes, hipStream_t st, bool time_it);
 
 // Trace frames[0] through surfs[0..n_surf) (n_surf <= 4); frames[k+1]
 // receives the frame after surface k.  d_errs[k]: per-ray surface error

@@ -96,12 +96,31 @@ size_t partial_bytes(int mode, const SumPlan &p, int n_tgt) {
     return (size_t)p.splits * comps * (size_t)n_tgt * sizeof(double);
 }
 
+SumPlan plan_cap_splits(const SumPlan &p, int n_src, int max_splits) {
+    if (p.splits <= max_splits || max_splits < 1) return p;
+    SumPlan q = p;
+    int chunk = (n_src + max_splits - 1) / max_splits;
+    q.chunk = (chunk + TILE - 1) / TILE * TILE;
+    q.splits = (n_src + q.chunk - 1) / q.chunk;
+    return q;
+}
+
 int launch_po(int mode, int precision, const void *packed, int n_src, const GridPtrs &ct,
               int n_tgt, double k, double epsilon, double t_dir, void *res, double *partial_ws,
               size_t ws_bytes, hipStream_t st, bool time_it) {
     PYPO_REQUIRE(n_src > 0 && n_tgt > 0, ERR_ARG, "empty source or target grid");
+    return launch_po_plan(plan_sum(mode, precision, n_src, n_tgt), mode, precision, packed, n_src, ct, n_tgt,
+                          k, epsilon, t_dir, res, partial_ws, ws_bytes, st, time_it);
+}
+
+int launch_po_plan(const SumPlan &p, int mode, int precision, const void *packed, int n_src,
+                   const GridPtrs &ct, int n_tgt, double k, double epsilon, double t_dir, void *res,
+                   double *partial_ws, size_t ws_bytes, hipStream_t st, bool time_it) {
+    PYPO_REQUIRE(n_src > 0 && n_tgt > 0, ERR_ARG, "empty source or target grid");
     PYPO_REQUIRE(mode >= PYPO_MODE_JM && mode <= PYPO_MODE_SCALAR, ERR_ARG, "unknown PO mode");
-    const SumPlan p = plan_sum(mode, precision, n_src, n_tgt);
+    PYPO_REQUIRE(p.tpl >= 1 && p.tpl <= 2 && p.tblocks == (n_tgt + TILE * p.tpl - 1) / (TILE * p.tpl) &&
+                     p.splits >= 1,
+                 ERR_ARG, "internal: bad target plan");
     PYPO_REQUIRE(partial_bytes(mode, p, n_tgt) <= ws_bytes, ERR_ARG, "partial workspace too small");
     PYPO_REQUIRE((long long)p.chunk * p.splits >= n_src && p.chunk % TILE == 0, ERR_ARG,
                  "internal: bad source split plan");

@@ -85,8 +85,11 @@ def generate_grid(refl, device="cuda"):
 
 
 def propagate(mode, src_grid, tgt_grid, src_values, src_refl, k, epsilon=1.0, exp="fwd",
-              precision="mixed", out=None):
-    """One PO propagation with every buffer in HBM; returns the result tensor."""
+              precision="mixed", out=None, no_alloc=False):
+    """One PO propagation with every buffer in HBM; returns the result tensor.
+    no_alloc=True runs pypo_hip_propagate_noalloc, which allocates no device
+    memory (its workspace is the library's fixed device scratch; `out` should
+    then be given, or this function allocates it with torch)."""
     lib = B.loadHIPlib()
     if mode not in N_OUT:
         raise ValueError(f"{mode} is not a valid propagation mode")
@@ -100,13 +103,12 @@ def propagate(mode, src_grid, tgt_grid, src_values, src_refl, k, epsilon=1.0, ex
     cs, ct = container(src_grid), container(tgt_grid)
     src_b, res_b = bundle(src_values), bundle(out)
     t_dir = -1.0 if exp == "fwd" else 1.0
-    rc = lib.pypo_hip_propagate_dev(B.MODES[mode], ctypes.cast(ctypes.byref(res_b), ctypes.c_void_p),
-                                    ctypes.byref(cs), ctypes.byref(ct),
-                                    ctypes.cast(ctypes.byref(src_b), ctypes.c_void_p),
-                                    int(src_refl["gmode"]), ncx, ncy, float(k), float(epsilon), t_dir,
-                                    B.PRECISIONS[precision], _stream())
+    fn = lib.pypo_hip_propagate_noalloc if no_alloc else lib.pypo_hip_propagate_dev
+    rc = fn(B.MODES[mode], ctypes.cast(ctypes.byref(res_b), ctypes.c_void_p), ctypes.byref(cs), ctypes.byref(ct),
+            ctypes.cast(ctypes.byref(src_b), ctypes.c_void_p), int(src_refl["gmode"]), ncx, ncy, float(k),
+            float(epsilon), t_dir, B.PRECISIONS[precision], _stream())
     if rc:
-        B.check(lib, "pypo_hip_propagate_dev")
+        B.check(lib, fn.__name__)
     return out
 
 
