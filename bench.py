@@ -194,6 +194,24 @@ def run_rt(torch, D):
     return X.size / dt, dt * 1e3, X.size, float(np.mean(kms))
 
 
+def run_system_chain(torch, w0, feed, dish, ff):
+    """The same mixed JM -> FF step through System.runPO, the reference's API:
+    the JM currents stay in device memory between the two calls.  Median of
+    3 synchronised steps after one warm step (which generates the grids)."""
+    from pypo_amd.system import System
+    s = System()
+    for name, refl in (("feed", feed), ("dish", dish), ("ff", ff)):
+        s.system[name] = dict(refl, name=name)
+    s.addCurrents("feedJM", feed_currents(w0), "feed", K)
+
+    def step():
+        s.runPO({"t_name": "dish", "mode": "JM", "s_current": "feedJM", "name_JM": "jm", "precision": "mixed"})
+        s.runPO({"t_name": "ff", "mode": "FF", "s_current": "jm", "name_EH": "ff", "precision": "mixed"})
+
+    step()
+    return median_time(step, 3)
+
+
 def run_sweep(torch, log):
     """Batched sweep: 64 frequencies (80-120 GHz) x 4 feed offsets = 256
     JM -> FF instances, dish grids 201^2 .. 401^2 (uneven), one 181^2 far
@@ -459,6 +477,11 @@ def main():
     peak_vector = FF_FLOPS / PAIR_TIME_FLOOR_VECTOR / 1e12
 
     res = {}
+    dts, spreads = run_system_chain(torch, w0, feed, dish, ff)
+    res["system_runPO_ms_per_step"] = round(dts * 1e3, 2)
+    res["system_runPO_ms_per_step_range"] = spreads
+    res["system_runPO_vs_device_step"] = round(dts / dt, 3)
+    log(f"System.runPO JM -> FF step {dts * 1e3:.1f} ms ({dts / dt:.3f} x the device-resident step)")
     issue_cycles = sum(ISSUE_CYCLES_PER_64_PAIRS.values())
     issue_ceiling = SIMDS * CLOCK_HZ * 64 / issue_cycles
     ff64 = jm64_host = None

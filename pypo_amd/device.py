@@ -137,7 +137,12 @@ def to_device(arr, device="cuda"):
 
 
 def currents_tensor(cur, device="cuda"):
-    """currents/fields object -> (12, n) tensor in c2Bundle order."""
+    """currents/fields object -> (12, n) tensor in c2Bundle order.  A result whose
+    values still live on the device hands over its own rows (no copy; callers
+    treat them as read-only)."""
+    rows = cur.device_rows() if hasattr(cur, "device_rows") else None
+    if rows is not None:
+        return rows
     comps = [np.asarray(c).ravel() for c in cur.components()]
     rows = [np.real(comps[0]), np.real(comps[1]), np.real(comps[2]),
             np.imag(comps[0]), np.imag(comps[1]), np.imag(comps[2]),
@@ -212,8 +217,58 @@ def interp_nearest(frame, values, grid):
 
 
 def frame_tensor(fr, device="cuda"):
-    """frame object -> (6, n) tensor."""
+    """frame object -> (6, n) tensor (the frame's own device rows when it has them)."""
+    rows = fr.device_rows() if hasattr(fr, "device_rows") else None
+    if rows is not None:
+        return rows
     return to_device(np.stack([np.ravel(getattr(fr, a)) for a in ("x", "y", "z", "dx", "dy", "dz")]), device)
+
+
+def scalar_tensor(sf, device="cuda"):
+    """scalarfield object -> (2, n) tensor rows Re, Im (its own device rows when it has them)."""
+    rows = sf.device_rows() if hasattr(sf, "device_rows") else None
+    if rows is not None:
+        return rows
+    s = np.ravel(np.asarray(sf.S))
+    return to_device(np.stack([np.real(s), np.imag(s)]), device)
+
+
+def grid_key(refl):
+    """Bytes that identify the grid a reflDict generates: every reflparams value
+    the generator reads (coefficients, limits, gridsize, flip, gmode, centre,
+    uv ellipticity and rotation, type, transformation, roughness and seed)."""
+    p = MU.fill_reflparams(refl)
+    k = p._keep
+    parts = [k[n].tobytes() for n in ("coeffs", "lxu", "lyv", "gcenter", "transf", "n_cells")]
+    parts.append(np.array([p.flip, p.gmode, p.type, p.rms_seed], dtype=np.int64).tobytes())
+    parts.append(np.array([p.ecc_uv, p.rot_uv, p.rms], dtype=np.float64).tobytes())
+    return b"".join(parts)
+
+
+class GridCache:
+    """Device grids of surfaces, kept between propagations and regenerated when
+    a surface's parameters change (keyed by grid_key); at most `max_entries`
+    grids are held, least recently used first out."""
+
+    def __init__(self, max_entries=8):
+        import collections
+        self.max_entries = max_entries
+        self._grids = collections.OrderedDict()
+
+    def get(self, refl):
+        key = grid_key(refl)
+        g = self._grids.get(key)
+        if g is None:
+            g = generate_grid(refl)
+            self._grids[key] = g
+            while len(self._grids) > self.max_entries:
+                self._grids.popitem(last=False)
+        else:
+            self._grids.move_to_end(key)
+        return g
+
+    def clear(self):
+        self._grids.clear()
 
 
 def tensor_to_frame(t):

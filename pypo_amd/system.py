@@ -147,6 +147,14 @@ class System:
         self.frames = {}
         self.assoc = {}
         self.override = override
+        self._grid_cache = None    # device grids of the surfaces, by their parameters
+
+    def _device_grid(self, refl):
+        """(7, n) device grid of a reflDict, generated once per parameter set."""
+        from . import device as D
+        if self._grid_cache is None:
+            self._grid_cache = D.GridCache()
+        return self._grid_cache.get(refl)
 
     # ---- surfaces ------------------------------------------------------------
     def _add(self, d, builder):
@@ -379,10 +387,16 @@ class System:
             raise RunPOError("\n".join(err))
 
     def runPO(self, runPODict):
-        """PO propagation (System.runPO, System.py:1113-1193); stores and returns the result."""
+        """PO propagation (System.runPO, System.py:1113-1193); stores and returns the result.
+
+        Runs on device-resident grids (cached per surface parameters) and keeps the
+        result in device memory: its values are copied to the host only when read,
+        and a following runPO, ray trace or figure of merit that takes it as input
+        reads it where it lies."""
         d = copy.copy(runPODict)
         self._check_runPODict(d)
-        if d["mode"] == "scalar":
+        mode = d["mode"]
+        if mode == "scalar":
             src = self.scalarfields[d["s_scalarfield"]]
             d["s_scalarfield"] = src
         else:
@@ -390,10 +404,9 @@ class System:
             d["s_current"] = src
         d["k"] = src.k
         source = self.system[src.surf]
-        target = self.system[d["t_name"]]
-        out = B.PyPO_HIPd(source, target, d)
         t, k = d["t_name"], d["k"]
-        mode = d["mode"]
+        target = self.system[t]
+        out = self._propagate(source, target, d)
         if mode == "JM":
             out.setMeta(t, k)
             self.currents[d["name_JM"]] = out
@@ -408,19 +421,47 @@ class System:
         elif mode == "EHP":
             out[0].setMeta(t, k)
             self.fields[d["name_EH"]] = out[0]
-            self.frames[d["name_P"]] = self._loadFramePoynt(out[1], t)
+            self.frames[d["name_P"]] = self._loadFramePoynt(out[1], target)
             self.assoc[t] = [d["name_EH"], d["name_P"]]
         else:
             out.setMeta(t, k)
             self.scalarfields[d["name_field"]] = out
         return out
 
-    def _loadFramePoynt(self, poynting, name_source):
-        """Frame of reflected Poynting vectors on their surface (System.py:3195-3215)."""
-        g = self.generateGrids(name_source)
-        n = poynting.x.size
-        return T.frame(n, g.x.ravel(), g.y.ravel(), g.z.ravel(), poynting.x.ravel(),
-                       poynting.y.ravel(), poynting.z.ravel())
+    def _propagate(self, source, target, d):
+        """One propagation on the GPU from a checked runPODict whose source entry
+        holds the source object; returns what PyPO_CPUd returns (BindCPU.py:82-220:
+        currents, fields, [currents, fields], [fields, rfield] or scalarfield),
+        with the values left in device memory."""
+        from . import device as D
+        mode = d["mode"]
+        vals = D.scalar_tensor(d["s_scalarfield"]) if mode == "scalar" else D.currents_tensor(d["s_current"])
+        rows = D.propagate(mode, self._device_grid(source), self._device_grid(target), vals, source, d["k"],
+                           epsilon=d["epsilon"], exp=d["exp"], precision=d["precision"])
+        shape = tuple(int(v) for v in target["gridsize"])
+        if mode == "JM":
+            return T.currents.on_device(rows, shape)
+        if mode in ("EH", "FF"):
+            return T.fields.on_device(rows, shape)
+        if mode == "JMEH":
+            return [T.currents.on_device(rows[:12], shape), T.fields.on_device(rows[12:], shape)]
+        if mode == "EHP":
+            return [T.fields.on_device(rows[:12], shape), T.rfield.on_device(rows[12:15], shape)]
+        return T.scalarfield.on_device(rows, shape)
+
+    def _loadFramePoynt(self, poynting, target):
+        """Frame of reflected Poynting vectors on their surface (System.py:3195-3215):
+        the surface's grid points with the Poynting directions, kept on the device
+        when the directions are there."""
+        rows = poynting.device_rows()
+        if rows is not None:
+            import torch
+            g = self._device_grid(target)
+            return T.frame.on_device(torch.cat([g[0:3], rows]).contiguous())
+        g = B.generate_grid(target)
+        n = np.asarray(poynting.x).size
+        return T.frame(n, g.x.ravel(), g.y.ravel(), g.z.ravel(), np.ravel(poynting.x), np.ravel(poynting.y),
+                       np.ravel(poynting.z))
 
     # ---- ray tracing -------------------------------------------------------------
     def _check_runRTDict(self, d):
@@ -455,13 +496,14 @@ class System:
             raise RunRTError("\n".join(err))
 
     def runRayTracer(self, runRTDict):
-        """Ray trace a frame onto a surface (System.runRayTracer, System.py:1531-1568)."""
+        """Ray trace a frame onto a surface (System.runRayTracer, System.py:1531-1568).
+        The traced frame stays in device memory until its rays are read."""
+        from . import device as D
         d = copy.copy(runRTDict)
         self._check_runRTDict(d)
         name_out = d["fr_out"]
-        d["fr_in"] = self.frames[d["fr_in"]]
-        d["t_name"] = self.system[d["t_name"]]
-        fr = B.RT_HIPd(d)
+        rays = D.frame_tensor(self.frames[d["fr_in"]])
+        fr = T.frame.on_device(D.trace([self.system[d["t_name"]]], rays, tol=d["tol"], t0=d["t0"])[0])
         fr.setMeta(self.calcRTcenter(fr), self.calcRTtilt(fr), np.eye(4))
         self.frames[name_out] = fr
         return fr
@@ -498,7 +540,8 @@ class System:
         c = _comp_index(comp)
         if c is None:
             raise ValueError("a field component is required")
-        return f, f[c]
+        rows = f.component_rows(c)     # (2, n) on the device while the field lives there
+        return f, (f[c] if rows is None else rows)
 
     def calcSpillover(self, name_field, comp, aperDict):
         """Spillover efficiency of a field component on an aperture of the field's surface."""
@@ -616,12 +659,17 @@ class System:
         if name_fr_in not in self.frames:
             raise RunRTError(f"Frame {name_fr_in} not in system.")
         field = self.fields[name_field]
-        grid = D.generate_grid(self._surface(name_target))
+        grid = self._device_grid(self._surface(name_target))
         shape = tuple(int(v) for v in self.system[name_target]["gridsize"])
         fr = self.frames[name_fr_in]
         ci = _comp_index(comp)
         if method == "nearest":
             out = D.interp_nearest(D.frame_tensor(fr), D.currents_tensor(field), grid)
+            if ci is None:
+                res = T.fields.on_device(out, shape)
+                res.setMeta(name_target, field.k)
+                self.fields[name_out] = res
+                return res
             comps = [c.reshape(shape) for c in D.tensor_to_complex6(out)]
         else:
             from scipy.interpolate import griddata
@@ -690,11 +738,10 @@ class System:
         field = self.fields[d["field_in"]]
         fr_t, f_t = D.hybrid(self.system[d["t_name"]], D.frame_tensor(self.frames[d["fr_in"]]),
                              D.currents_tensor(field), field.k, start=d["start"], tol=d["tol"], t0=d["t0"])
-        fr_out = D.tensor_to_frame(fr_t)
+        fr_out = T.frame.on_device(fr_t)
         fr_out.setMeta(self.calcRTcenter(fr_out), self.calcRTtilt(fr_out), np.eye(4))
         self.frames[d["fr_out"]] = fr_out
-        shape = np.asarray(field.Ex).shape
-        carried = T.fields(*[c.reshape(shape) for c in D.tensor_to_complex6(f_t)])
+        carried = T.fields.on_device(f_t, field.shape)
         carried.setMeta(d["t_name"], field.k)
         self.fields[d["field_out"]] = carried
         if d["interp"]:

@@ -49,7 +49,7 @@ def test_autoConverge_steps(monkeypatch):
         a = np.full((g0, g1), 1.0 / (g0 * g1))
         return T.reflGrids(a, a, a, a, a, a, a)
 
-    monkeypatch.setattr(B, "PyPO_HIPd", fake_po)
+    monkeypatch.setattr(System, "_propagate", lambda self, src, tgt, d: fake_po(src, tgt, d))
     monkeypatch.setattr(B, "generate_grid", fake_grid)
     out = s.autoConverge("beam", "scat")
     assert type(out) is np.ndarray
@@ -78,7 +78,7 @@ def test_autoConverge_max_iter(monkeypatch):
         a = np.full((g0, g1), 1.0)
         return T.reflGrids(a, a, a, a, a, a, a)
 
-    monkeypatch.setattr(B, "PyPO_HIPd", fake_po)
+    monkeypatch.setattr(System, "_propagate", lambda self, src, tgt, d: fake_po(src, tgt, d))
     monkeypatch.setattr(B, "generate_grid", fake_grid)
     # P grows with the grid and never settles: stops after max_iter steps
     out = s.autoConverge("beam", "scat", add=1, max_iter=4, patch_size=0.5)
@@ -104,7 +104,7 @@ def test_convergeOnTarget_steps(monkeypatch):
         s0, s1 = gridsize(source)
         return T.scalarfield(np.full((g0, g1), 1 + 4 / s0 ** 2 + 1 / s1 ** 2, dtype=complex))
 
-    monkeypatch.setattr(B, "PyPO_HIPd", fake_po)
+    monkeypatch.setattr(System, "_propagate", lambda self, src, tgt, d: fake_po(src, tgt, d))
     out = s.convergeOnTarget("beam", "scat", "tgt")
     assert list(out) == [22, 11]
     assert list(s.system["scat"]["gridsize"]) == [22, 11]
@@ -144,7 +144,7 @@ def test_converge_loops_with_checker_propagation(port, monkeypatch):
     """Dry run of both loops on the CPU checker: they terminate on a real
     propagation and give grids the GPU test below must reproduce."""
     s = _dish_system()
-    monkeypatch.setattr(B, "PyPO_HIPd", _checker_po(port))
+    monkeypatch.setattr(System, "_propagate", lambda self, src, tgt, d: _checker_po(port)(src, tgt, d))
     monkeypatch.setattr(B, "generate_grid", lambda r, t=True, sp=True: OB.generate_grid(port, r, t, sp))
     a = s.autoConverge("beam", "dish")
     assert a.dtype.kind == "i" and list(a) == [189, 189]
@@ -166,7 +166,7 @@ def test_converge_loops_gpu(hip, port, monkeypatch):
 
     # the same loops on the CPU checker's propagation
     r = _dish_system()
-    monkeypatch.setattr(B, "PyPO_HIPd", _checker_po(port))
+    monkeypatch.setattr(System, "_propagate", lambda self, src, tgt, d: _checker_po(port)(src, tgt, d))
     monkeypatch.setattr(B, "generate_grid", lambda g, t=True, sp=True: OB.generate_grid(port, g, t, sp))
     assert list(r.autoConverge("beam", "dish")) == list(gpu_auto)
     r.system["dish"]["gridsize"] = np.array([25, 24])

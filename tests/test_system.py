@@ -195,3 +195,38 @@ def test_system_beams_and_frames(hip, port):
     assert group_err(flat(out), flat(ref_jm)) <= 1e-10
     lens = s.calcRayLen("tube", "tube")
     assert np.all(lens[0] == 0) and len(lens) == 2
+
+
+@pytest.mark.gpu
+def test_runPO_chains_on_device(hip, monkeypatch):
+    """JM then FF through System: the JM currents stay in device memory and
+    feed the FF call without any device-to-host copy; both results are
+    bit-identical to the drop-in path's (same kernels on the same inputs);
+    reading a component copies the values to the host once."""
+    from pypo_amd import bind as B
+    s = make_system()
+    k = s.currents["feedJM"].k
+    copies = []
+    real = T._DeviceBacked._to_host
+
+    def spy(self):
+        if self.device_rows() is not None:
+            copies.append(type(self).__name__)
+        return real(self)
+
+    monkeypatch.setattr(T._DeviceBacked, "_to_host", spy)
+    s.runPO({"t_name": "dish", "mode": "JM", "s_current": "feedJM", "name_JM": "dJM"})
+    s.runPO({"t_name": "ff", "mode": "FF", "s_current": "dJM", "name_EH": "ffEH"})
+    assert copies == []
+    assert s.currents["dJM"].device_rows() is not None and s.fields["ffEH"].device_rows() is not None
+    jm_host = B.PyPO_HIPd(s.system["feed"], s.system["dish"], {"mode": "JM", "k": k, "s_current": s.currents["feedJM"]})
+    ff_host = B.PyPO_HIPd(s.system["dish"], s.system["ff"], {"mode": "FF", "k": k, "s_current": jm_host})
+    np.testing.assert_array_equal(flat(s.fields["ffEH"]), flat(ff_host))
+    assert copies == ["fields"] and s.fields["ffEH"].device_rows() is None
+    np.testing.assert_array_equal(flat(s.currents["dJM"]), flat(jm_host))
+    assert copies == ["fields", "currents"]
+    # a surface whose parameters change gets a new grid
+    s.system["ff"]["lims_Az"] = np.array([-2.0, 2.0])
+    s.runPO({"t_name": "ff", "mode": "FF", "s_current": "dJM", "name_EH": "ffEH2"})
+    ff2 = B.PyPO_HIPd(s.system["dish"], s.system["ff"], {"mode": "FF", "k": k, "s_current": jm_host})
+    np.testing.assert_array_equal(flat(s.fields["ffEH2"]), flat(ff2))
