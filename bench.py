@@ -238,12 +238,15 @@ def run_sweep(torch, log):
     pairs = info["pairs_jm"] + info["pairs_ff"]
     worst = max(table, key=lambda r: r["seconds"])
     sw.close()
+    table_ms = [dict(AT.describe(r["params"]), ms=round(r["seconds"] * 1e3, 2),
+                     range_ms=[round(r["min_s"] * 1e3, 2), round(r["max_s"] * 1e3, 2)]) for r in table]
     log(f"sweep {len(feeds)} instances: {dt * 1e3:.1f} ms/run ({pairs / dt:.3e} pairs/s), tuned {best} "
         f"in {tune_s:.1f} s")
     return {"instances": len(feeds), "pairs_per_run": pairs, "ms_per_run": round(dt * 1e3, 2),
             "pairs_per_s": pairs / dt, "launch": "hipGraph replay of 2 persistent stages",
-            "tuned": best, "tuning_s": round(tune_s, 1), "setup_s": round(setup_s, 1),
+            "tuned": AT.describe(best), "tuning_s": round(tune_s, 1), "setup_s": round(setup_s, 1),
             "slowest_candidate_ms": round(worst["seconds"] * 1e3, 1),
+            "tuning_table": table_ms, "tuning_timing": "median of 3 synchronised replays per candidate",
             "workload": "64 freqs 80-120 GHz x offsets (0,2,-2,4) mm; paraboloid D=1000 f=400 uv "
                         "201^2..401^2; feed 25x25; AoE 181^2 +-1deg"}
 

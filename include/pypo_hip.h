@@ -259,8 +259,9 @@ int pypo_hip_trace_dev(const reflparams *surfs, int n_surf, cframe *frames,
  * dish surface, wavenumber k (1/mm) and the feed currents (host arrays of the
  * feed grid's size).  The JM step (feed -> dish, PEC currents) and the FF
  * step (dish -> ff) of every instance run as one persistent launch each over
- * a static work list sorted by cost; mixed precision.  tpl (targets per lane,
- * 1 or 2), chunk_tiles (256-source tiles per work item) and wgs_per_cu
+ * a static work list sorted by cost; mixed precision.  tpl (targets per lane:
+ * 1 or 2, or 4 = 16 M-blocks of the matrix-core far field at one wave per
+ * SIMD), chunk_tiles (256-source tiles per work item) and wgs_per_cu
  * (persistent workgroups per CU) shape the launch; 0 picks defaults.
  * use_graph captures the whole run into a hipGraph that pypo_hip_sweep_run
  * replays.  No counterpart in the reference (its batch loops are Python). */

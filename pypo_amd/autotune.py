@@ -1,10 +1,14 @@
 """Autotuner for the persistent sweep launches.
 
-Times every candidate launch shape -- targets per lane (the target-block
-shape: 256 x TPL targets per workgroup), source tiles per work item, and
-persistent workgroups per CU -- on the actual sweep, keeps the fastest, and
-caches the choice in a JSON file keyed by the device and a signature of the
-sweep (instance count, pair counts).  Cache path: $PYPO_AMD_TUNE_CACHE or
+Times every candidate launch shape on the actual sweep, keeps the fastest,
+and caches the choice with the whole timing table in a JSON file keyed by
+the device and a signature of the sweep (instance count, pair counts).
+Dimensions: targets per lane (TPL: the far field's target block is 256 TPL
+targets per workgroup, its matrix-core kernel runs 4 TPL M-blocks of 16
+targets per wave; TPL 4 is the 16-M-block kernel at one wave per SIMD),
+256-source tiles per work item, and persistent workgroups per CU.  Each
+candidate is timed as the median of `reps` individually synchronised
+replays after one warm run.  Cache path: $PYPO_AMD_TUNE_CACHE or
 ~/.cache/pypo_amd/autotune.json.
 """
 
@@ -18,7 +22,7 @@ CANDIDATES = [
     for tpl in (1, 2)
     for ch in (64, 160, 4096)
     for w in (2, 4, 6)
-]
+] + [{"tpl": 4, "chunk_tiles": ch, "wgs_per_cu": 1} for ch in (64, 160, 4096)]
 
 
 def cache_path():
@@ -60,15 +64,28 @@ def _store(db):
         pass   # a read-only home is not an error: the tuning is just not remembered
 
 
-def _time_run(sweep, reps=2):
+def _time_run(sweep, reps=3):
+    """Median, min and max seconds of `reps` individually synchronised runs."""
     import torch
     sweep.run()
     torch.cuda.synchronize()
-    t = time.perf_counter()
+    ts = []
     for _ in range(reps):
+        t = time.perf_counter()
         sweep.run()
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t) / reps
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+    ts.sort()
+    return ts[len(ts) // 2], ts[0], ts[-1]
+
+
+def describe(params):
+    """The launch shape of a candidate, with the quantities it fixes."""
+    d = dict(params)
+    d["m_blocks"] = 4 * int(params["tpl"])
+    d["targets_per_workgroup"] = 256 * int(params["tpl"])
+    d["lds_tile_sources"] = 256
+    return d
 
 
 def tune(sweep, candidates=None, use_cache=True, use_graph=True, log=None):
@@ -83,10 +100,10 @@ def tune(sweep, candidates=None, use_cache=True, use_graph=True, log=None):
     table = []
     for c in candidates or CANDIDATES:
         sweep.plan(use_graph=False, **c)
-        dt = _time_run(sweep)
-        table.append({"params": c, "seconds": dt})
+        dt, lo, hi = _time_run(sweep)
+        table.append({"params": c, "seconds": dt, "min_s": lo, "max_s": hi})
         if log:
-            log(f"autotune {c}: {dt * 1e3:.2f} ms")
+            log(f"autotune {c}: {dt * 1e3:.2f} ms (range {lo * 1e3:.2f}..{hi * 1e3:.2f})")
     best = min(table, key=lambda r: r["seconds"])
     sweep.plan(use_graph=use_graph, **best["params"])
     if use_cache:

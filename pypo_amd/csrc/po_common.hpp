@@ -7,6 +7,16 @@ namespace pypo {
 
 constexpr int TILE = 256;   // sources per LDS tile == threads per workgroup
 
+// Packed record slot of grid point i (i = xu ncy + yv): serpentine order, odd
+// grid rows reversed, when the grid is ncx x ncy, else i.  The sums do not
+// depend on the order; consecutive records stay grid neighbours across row
+// ends, which the matrix far field's follower phase relies on.
+__device__ __forceinline__ int packed_slot(int i, int n, int ncx, int ncy) {
+    if (ncy < 2 || (long long)ncx * ncy != n) return i;
+    const int xu = i / ncy, yv = i - xu * ncy;
+    return (xu & 1) ? xu * ncy + (ncy - 1 - yv) : i;
+}
+
 // LDS tile staging: a tile of TILE records of REC_U4 16-byte words, loaded
 // one record per lane.  Past the end of the split the lane writes a null
 // record instead (all weights zero, position 1e7 mm away from anything) so

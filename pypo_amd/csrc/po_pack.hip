@@ -18,14 +18,6 @@
 namespace pypo {
 namespace {
 
-// Record slot of grid point i (i = xu ncy + yv): odd rows reversed when the
-// grid is ncx x ncy, else i.
-__device__ __forceinline__ int packed_slot(int i, int n, int ncx, int ncy) {
-    if (ncy < 2 || (long long)ncx * ncy != n) return i;
-    const int xu = i / ncy, yv = i - xu * ncy;
-    return (xu & 1) ? xu * ncy + (ncy - 1 - yv) : i;
-}
-
 __global__ __launch_bounds__(256) void k_pack32(GridPtrs cs, int n, Bundle12 cur, int gmode,
                                                 int ncx, int ncy, SrcRec32 *__restrict__ out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

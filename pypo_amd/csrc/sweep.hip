@@ -65,6 +65,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
+// Sixteen M-blocks (1024 targets per item) at one wave per SIMD: twice the
+// targets per staged tile, half the waves to hide latency with.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void k_sweep_ff_mfma16(
+    const StageInst *__restrict__ insts, const SweepItem *__restrict__ items, int n_items) {
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const SweepItem item = items[it];
+        const StageInst &in = insts[item.inst];
+        mfma::ff_mfma_body<16>(in.src, in.meta, item.s0, item.s1, item.tblock * 64 * 16, in.tx, in.ty,
+                               in.n_tgt, in.rev_scale, in.partial + (size_t)item.chunk * 12 * in.n_tgt);
+        __syncthreads();   // the next item reuses the LDS tiles
+    }
+}
+
 template <int TPL>
 __global__ __launch_bounds__(256) void k_sweep_ff(const StageInst *__restrict__ insts,
                                                   const SweepItem *__restrict__ items, int n_items) {
@@ -285,7 +298,7 @@ __global__ __launch_bounds__(256) void k_sweep_pack(const PackInst *__restrict__
         r.cm[2 * a + 1] = (float)(wa * in.cur[(9 + a) * n + i]);
     }
     r.pad1[0] = 0.f; r.pad1[1] = 0.f;
-    in.rec[i] = r;
+    in.rec[packed_slot(i, in.n, in.ncx, in.ncy)] = r;
 }
 
 // grid: (max tiles, n_inst)
@@ -407,7 +420,8 @@ int launch_stage_sum(bool far, const Sweep *s, hipStream_t st) {
     if (n == 0) return 0;
     if (far) {
         if (ff_kernel() == PYPO_FF_MATRIX) {
-            if (s->prm.tpl == 2) hipLaunchKernelGGL(k_sweep_ff_mfma<8>, dim3(grid), dim3(TILE), 0, st, ins, items, n);
+            if (s->prm.tpl == 4) hipLaunchKernelGGL(k_sweep_ff_mfma16, dim3(grid), dim3(TILE), 0, st, ins, items, n);
+            else if (s->prm.tpl == 2) hipLaunchKernelGGL(k_sweep_ff_mfma<8>, dim3(grid), dim3(TILE), 0, st, ins, items, n);
             else hipLaunchKernelGGL(k_sweep_ff_mfma<4>, dim3(grid), dim3(TILE), 0, st, ins, items, n);
         } else if (s->prm.tpl == 2) {
             hipLaunchKernelGGL(k_sweep_ff<2>, dim3(grid), dim3(TILE), 0, st, ins, items, n);
@@ -445,7 +459,9 @@ int enqueue_run(const Sweep *s, hipStream_t st) {
 
 // (Re)build work lists, partial buffers and, if asked, the captured graph.
 int sweep_plan(Sweep *s, SweepPlanParams prm, bool use_graph, hipStream_t st) {
-    if (prm.tpl != 1 && prm.tpl != 2) prm.tpl = 2;
+    // targets per lane: 1, 2, or 4 (16 M-blocks; matrix-core far field only)
+    if (prm.tpl != 1 && prm.tpl != 2 && prm.tpl != 4) prm.tpl = 2;
+    if (prm.tpl == 4 && ff_kernel() != PYPO_FF_MATRIX) prm.tpl = 2;
     if (prm.chunk_tiles < 1) prm.chunk_tiles = 1 << 20;
     if (prm.wgs_per_cu < 1) prm.wgs_per_cu = 2;
     PYPO_REQUIRE(prm.wgs_per_cu <= 8, ERR_ARG, "sweep: wgs_per_cu must be 1..8");
