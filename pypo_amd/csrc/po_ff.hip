@@ -9,7 +9,9 @@
 //
 // Per-pair work (mixed precision): the phase in revolutions
 // rev = x ux + y uy + z uz (FP64, u = r * (-t k / 2 pi)), its fraction
-// (v_fract_f64), FP32 sin/cos of the fraction, then FMAs.  Per tile (256
+// (v_fract_f64), FP32 sin/cos of the fraction, then FMAs; in tiles where the
+// follower bound holds, one such FP64 phase per group of eight neighbouring
+// records and FP32 offsets for the other seven (ff_tile_follow).  Per tile (256
 // sources) two uniform decisions pick the inner loop: the shadow test is
 // skipped when the tile's normal cone and the workgroup's direction cone
 // prove n.r > 0 for every pair, and the M terms are skipped when the tile
@@ -21,6 +23,100 @@
 namespace pypo {
 namespace {
 
+// Largest bound on a follower phase sum (revolutions): FP32 rounds such a sum
+// to 2^-24 * 8 = 5e-7 revolutions at worst, and v_sin/v_cos reduce it
+// themselves.
+constexpr float kFollowRev = 8.f;
+
+// The per-tile loop with the follower phase (as in the matrix-core kernel,
+// po_ff_mfma.hpp): records come in groups of kGroup grid neighbours (the
+// packer writes them in serpentine order); the fourth of a group (the
+// leader, at most four records from any other) gets the FP64 phase, the
+// other seven
+//   f = f_leader + d . u        (three FP32 FMAs)
+// with d = (r_s - r_leader) * rev_scale in FP32, written into the record's
+// padding (pad0, pad1) when the tile was staged, and u the FP32 direction.
+// Taken only when every follower's 1 + sum_i |d_i| ub_i <= kFollowRev.
+// Measured on the large far field with PEC currents (kernel only): no
+// follower form 121.4 ms, groups of 4 116.3 ms, groups of 8 108.1 ms,
+// groups of 16 107.9 ms (twice the offsets for 0.2 % more).
+constexpr int kGroup = 8, kLeader = 3;
+__device__ __forceinline__ int group_leader(int lane) { return (lane & ~(kGroup - 1)) | kLeader; }
+template <int TPL, bool HAS_M, bool MASK>
+__device__ __forceinline__ void ff_tile_follow(const SrcRec32 *tile, const double (&rx)[TPL],
+                                               const double (&ry)[TPL], const double (&rz)[TPL],
+                                               const float (&fx)[TPL], const float (&fy)[TPL],
+                                               const float (&fz)[TPL], float (&acc)[TPL][12]) {
+#pragma unroll 1
+    for (int s0 = 0; s0 < TILE; s0 += kGroup) {
+        float f_lead[TPL];
+#pragma unroll
+        for (int k = 0; k < kGroup; k++) {
+            const int js = k == 0 ? kLeader : (k <= kLeader ? k - 1 : k);   // the leader first
+            const SrcRec32 &r = tile[s0 + js];
+            float cj[6], cm[6];
+#pragma unroll
+            for (int q = 0; q < 6; q++) cj[q] = r.cj[q];
+            if (HAS_M) {
+#pragma unroll
+                for (int q = 0; q < 6; q++) cm[q] = r.cm[q];
+            }
+#pragma unroll
+            for (int j = 0; j < TPL; j++) {
+                float f;
+                if (js == kLeader) {
+                    const double rev = __builtin_fma(r.z, rz[j], __builtin_fma(r.y, ry[j], r.x * rx[j]));
+                    f = (float)__builtin_amdgcn_fract(rev);
+                    f_lead[j] = f;
+                } else {
+                    f = __builtin_fmaf(r.pad1[1], fz[j], __builtin_fmaf(r.pad1[0], fy[j],
+                                                                        __builtin_fmaf(r.pad0, fx[j], f_lead[j])));
+                }
+                float sn, cs;
+                sincos_rev_f32(f, sn, cs);
+                if (MASK) {
+                    const float m = __builtin_fmaf(r.nz, fz[j], __builtin_fmaf(r.ny, fy[j], r.nx * fx[j]));
+                    cs = (m < 0.f) ? 0.f : cs;
+                    sn = (m < 0.f) ? 0.f : sn;
+                }
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    float &ar = acc[j][2 * q], &ai = acc[j][2 * q + 1];
+                    ar = __builtin_fmaf(cj[2 * q], cs, ar);
+                    ar = __builtin_fmaf(-cj[2 * q + 1], sn, ar);
+                    ai = __builtin_fmaf(cj[2 * q], sn, ai);
+                    ai = __builtin_fmaf(cj[2 * q + 1], cs, ai);
+                }
+                if (HAS_M) {
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        float &ar = acc[j][6 + 2 * q], &ai = acc[j][7 + 2 * q];
+                        ar = __builtin_fmaf(cm[2 * q], cs, ar);
+                        ar = __builtin_fmaf(-cm[2 * q + 1], sn, ar);
+                        ai = __builtin_fmaf(cm[2 * q], sn, ai);
+                        ai = __builtin_fmaf(cm[2 * q + 1], cs, ai);
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int TPL, bool HAS_M>
+__device__ __forceinline__ void ff_tile_vector(bool mask, bool follow, const SrcRec32 *tile,
+                                               const double (&rx)[TPL], const double (&ry)[TPL],
+                                               const double (&rz)[TPL], const float (&fx)[TPL],
+                                               const float (&fy)[TPL], const float (&fz)[TPL],
+                                               float (&acc)[TPL][12]) {
+    if (follow) {
+        if (mask) ff_tile_follow<TPL, HAS_M, true>(tile, rx, ry, rz, fx, fy, fz, acc);
+        else ff_tile_follow<TPL, HAS_M, false>(tile, rx, ry, rz, fx, fy, fz, acc);
+    } else {
+        if (mask) ff_tile_mixed<TPL, HAS_M, true>(tile, rx, ry, rz, fx, fy, fz, acc);
+        else ff_tile_mixed<TPL, HAS_M, false>(tile, rx, ry, rz, fx, fy, fz, acc);
+    }
+}
+
 template <int TPL>
 __global__ __launch_bounds__(256) void k_ff_mixed(const SrcRec32 *__restrict__ src,
                                                   const TileMeta *__restrict__ meta, int n_src,
@@ -29,7 +125,8 @@ __global__ __launch_bounds__(256) void k_ff_mixed(const SrcRec32 *__restrict__ s
                                                   double rev_scale, double *__restrict__ partial) {
     __shared__ SrcRec32 tile[TILE];
     __shared__ float scratch[4];
-    const int tid = threadIdx.x;
+    __shared__ int follow_ok[4];   // per wave: every follower of its 64 records within the bound
+    const int tid = threadIdx.x, lane = tid & 63;
     double rx[TPL], ry[TPL], rz[TPL];
     float fx[TPL], fy[TPL], fz[TPL];
     Cone cone;
@@ -69,6 +166,14 @@ __global__ __launch_bounds__(256) void k_ff_mixed(const SrcRec32 *__restrict__ s
     for (int j = 0; j < TPL; j++)
 #pragma unroll
         for (int q = 0; q < 12; q++) { acc[j][q] = 0.f; tot[j][q] = 0.0; }
+    // per-component bound on |u| over the workgroup's targets: |a_i| + sin(theta)
+    float ubx = 1.f, uby = 1.f, ubz = 1.f;
+    if (cone.cos_half > 0.f) {
+        const float sn = sqrtf(fmaxf(0.f, 1.f - cone.cos_half * cone.cos_half)) + 1e-3f;
+        ubx = fminf(1.f, fabsf(cone.ax) + sn);
+        uby = fminf(1.f, fabsf(cone.ay) + sn);
+        ubz = fminf(1.f, fabsf(cone.az) + sn);
+    }
 
     const int s_begin = blockIdx.y * chunk;
     const int s_end = min(n_src, s_begin + chunk);
@@ -76,19 +181,32 @@ __global__ __launch_bounds__(256) void k_ff_mixed(const SrcRec32 *__restrict__ s
     TileLoader<6> ld;
     ld.load(g, s_begin + tid, s_end);
     for (int base = s_begin; base < s_end; base += TILE) {
-        __syncthreads();
-        ld.store(reinterpret_cast<uint4 *>(tile), tid);
-        __syncthreads();
+        {
+            // this lane's follower offset from its group's leader (same wave), in revolutions,
+            // written into the padding of its record's LDS copy
+            const SrcRec32 r = __builtin_bit_cast(SrcRec32, ld.buf);
+            const int ls = group_leader(lane);
+            const float ddx = (float)((r.x - __shfl(r.x, ls, 64)) * rev_scale);
+            const float ddy = (float)((r.y - __shfl(r.y, ls, 64)) * rev_scale);
+            const float ddz = (float)((r.z - __shfl(r.z, ls, 64)) * rev_scale);
+            const float bound = 1.f + fabsf(ddx) * ubx + fabsf(ddy) * uby + fabsf(ddz) * ubz;
+            const bool ok = (lane == ls) || (bound <= kFollowRev);   // NaN: not ok
+            const unsigned long long all_ok = __ballot(ok);
+            ld.buf[2].y = __float_as_uint(ddx);   // pad0
+            ld.buf[5].z = __float_as_uint(ddy);   // pad1[0]
+            ld.buf[5].w = __float_as_uint(ddz);   // pad1[1]
+            __syncthreads();
+            ld.store(reinterpret_cast<uint4 *>(tile), tid);
+            if (lane == 0) follow_ok[tid >> 6] = all_ok == ~0ull;
+            __syncthreads();
+        }
         if (base + TILE < s_end) ld.load(g, base + TILE + tid, s_end);
         const TileMeta m = meta[base / TILE];
         const bool mask = cones_may_shadow(m.ax, m.ay, m.az, m.cos_half, cone);
-        if (m.has_m) {
-            if (mask) ff_tile_mixed<TPL, true, true>(tile, rx, ry, rz, fx, fy, fz, acc);
-            else ff_tile_mixed<TPL, true, false>(tile, rx, ry, rz, fx, fy, fz, acc);
-        } else {
-            if (mask) ff_tile_mixed<TPL, false, true>(tile, rx, ry, rz, fx, fy, fz, acc);
-            else ff_tile_mixed<TPL, false, false>(tile, rx, ry, rz, fx, fy, fz, acc);
-        }
+        const bool follow =
+            __builtin_amdgcn_readfirstlane(follow_ok[0] & follow_ok[1] & follow_ok[2] & follow_ok[3]);
+        if (m.has_m) ff_tile_vector<TPL, true>(mask, follow, tile, rx, ry, rz, fx, fy, fz, acc);
+        else ff_tile_vector<TPL, false>(mask, follow, tile, rx, ry, rz, fx, fy, fz, acc);
 #pragma unroll
         for (int j = 0; j < TPL; j++)
 #pragma unroll

@@ -352,8 +352,8 @@ def test_kernel_timing_api(hip):
 
 @pytest.mark.parametrize("seed", range(24))
 def test_ff_matrix_random_cases(hip, port, seed):
-    """Seeded random far-field cases for the matrix kernel against the CPU
-    checker: xy or uv source grids of random shape (partial last tiles, row
+    """Seeded random far-field cases for the matrix and the vector kernel
+    against the CPU checker: xy or uv source grids of random shape (partial last tiles, row
     lengths that do and do not divide by four), random wavelengths (follower
     and FP64 phase tiles), random far-field cones from +-0.1 to +-80 deg,
     random J and M currents, sometimes a tilted surface."""
@@ -375,12 +375,16 @@ def test_ff_matrix_random_cases(hip, port, seed):
     if rng.random() < 0.5:
         comps[3:] = [np.zeros((nx, ny), complex)] * 3
     cur = T.currents(*comps)
-    B.set_ff_kernel("matrix")
-    out = B.PyPO_HIPd(src, ff, {"mode": "FF", "k": k, "s_current": cur, "precision": "mixed"})
     ref, _, _ = OB.propagate(port, "FF", src, ff, cur, k)
-    err = group_err(flat(out), flat(ref))
-    print(f"seed {seed}: {nx}x{ny} {src['gmode']} lam {lam:.3g} cone {deg:.3g} deg: {err:.2e}")
-    assert err <= TOL["mixed"]
+    try:
+        for kind in ("matrix", "vector"):
+            B.set_ff_kernel(kind)
+            out = B.PyPO_HIPd(src, ff, {"mode": "FF", "k": k, "s_current": cur, "precision": "mixed"})
+            err = group_err(flat(out), flat(ref))
+            print(f"seed {seed}: {nx}x{ny} {src['gmode']} lam {lam:.3g} cone {deg:.3g} deg, {kind}: {err:.2e}")
+            assert err <= TOL["mixed"]
+    finally:
+        B.set_ff_kernel("matrix")
 
 
 @pytest.mark.parametrize("seed", range(15))
