@@ -343,9 +343,13 @@ def cpu_chain(feed, dish, cur_feed, sub_rows, threads):
     nrow = len(sub_rows)
     sub = G.plane({"gmode": "AoE", "lims_Az": [-FF_LIM, FF_LIM], "lims_El": [-FF_LIM, FF_LIM],
                    "gridsize": [nrow, N_FF]})
-    t = time.perf_counter()
-    out, _, _ = OB.propagate(lib, "FF", dish, sub, jm, K, threads)
-    dt = time.perf_counter() - t
+    # median of 3 runs (BASELINE.md's protocol for CPU jobs under a minute)
+    runs = []
+    for _ in range(3):
+        t = time.perf_counter()
+        out, _, _ = OB.propagate(lib, "FF", dish, sub, jm, K, threads)
+        runs.append(time.perf_counter() - t)
+    dt = float(np.median(runs))
     vals = np.stack([c.ravel() for c in out.components()])
     pairs = N_DISH * N_DISH * nrow * N_FF
     half = G.plane({"gmode": "AoE", "lims_Az": [-FF_LIM, FF_LIM], "lims_El": [-FF_LIM, FF_LIM],
@@ -359,7 +363,8 @@ def cpu_chain(feed, dish, cur_feed, sub_rows, threads):
             "kind": "reference" if kind == "ref" else "port",
             "sample": f"FF rows {sub_rows[0]}..{sub_rows[-1]} step {CPU_ROW_STRIDE} of the 501x501 "
                       f"far field ({nrow * N_FF} targets x {N_DISH * N_DISH} sources = {pairs:.3e} "
-                      f"pairs), one native call incl. its grid generation, {dt:.1f} s",
+                      f"pairs), one native call incl. its grid generation, median of 3: {dt:.2f} s "
+                      f"(runs {', '.join(f'{r:.2f}' for r in runs)} s)",
             "threads_from": "affinity mask, capped by OMP_NUM_THREADS", **counts,
             "compile_flags": "g++ -O3 -pthread -fPIC (CMakeLists.txt:11 release flags)",
             "jm_full_dish_s": round(jm_s, 2),
