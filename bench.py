@@ -425,7 +425,7 @@ def main():
     if args.profile_sweep:
         from pypo_amd import sweep as SW
         feeds, dishes, ks, curs, ff_s, _ = SW.frequency_offset_sweep()
-        sw = SW.Sweep(feeds, dishes, ks, curs, ff_s, tpl=2, chunk_tiles=4096, wgs_per_cu=4, use_graph=True)
+        sw = SW.Sweep(feeds, dishes, ks, curs, ff_s, tpl=2, chunk_tiles=4096, wgs_per_cu=64, use_graph=True)
         for _ in range(3):
             sw.run()
         torch.cuda.synchronize()
@@ -450,6 +450,11 @@ def main():
     if args.profile:
         step("mixed")
         step("fp64")
+        B.set_ff_kernel("vector")    # the same mixed step with the far field on the vector ALUs
+        try:
+            step("mixed")
+        finally:
+            B.set_ff_kernel("matrix")
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = src.clone()            # calibration copy: 512 MiB read + 512 MiB written
         torch.cuda.synchronize()

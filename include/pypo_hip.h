@@ -262,7 +262,9 @@ int pypo_hip_trace_dev(const reflparams *surfs, int n_surf, cframe *frames,
  * a static work list sorted by cost; mixed precision.  tpl (targets per lane:
  * 1 or 2, or 4 = 16 M-blocks of the matrix-core far field at one wave per
  * SIMD), chunk_tiles (256-source tiles per work item) and wgs_per_cu
- * (persistent workgroups per CU) shape the launch; 0 picks defaults.
+ * (workgroups per CU in each launch, 1..4096, never more than the stage's
+ * work items) shape the launch; 0 picks defaults (2, whole feeds/dishes,
+ * 64).
  * use_graph captures the whole run into a hipGraph that pypo_hip_sweep_run
  * replays.  No counterpart in the reference (its batch loops are Python). */
 typedef struct pypo_sweep_desc {

@@ -21,8 +21,8 @@ CANDIDATES = [
     {"tpl": tpl, "chunk_tiles": ch, "wgs_per_cu": w}
     for tpl in (1, 2)
     for ch in (64, 160, 4096)
-    for w in (2, 4, 6)
-] + [{"tpl": 4, "chunk_tiles": ch, "wgs_per_cu": 1} for ch in (64, 160, 4096)]
+    for w in (4, 16, 64)
+] + [{"tpl": 4, "chunk_tiles": ch, "wgs_per_cu": w} for ch in (160, 4096) for w in (2, 64)]
 
 
 def cache_path():

@@ -413,11 +413,15 @@ void build_items(const std::vector<StageInst> &st, int tpl, int chunk_tiles,
 }
 
 int launch_stage_sum(bool far, const Sweep *s, hipStream_t st) {
-    const int grid = std::max(1, s->n_cu * s->prm.wgs_per_cu);
     const StageInst *ins = far ? s->d_ff : s->d_jm;
     const SweepItem *items = far ? s->d_items_ff : s->d_items_jm;
     const int n = far ? s->n_items_ff : s->n_items_jm;
     if (n == 0) return 0;
+    // More workgroups than fit at once is deliberate: the dispatcher starts
+    // each as a slot frees, so with the list sorted longest first a grid of
+    // one workgroup per item schedules the items longest-first onto free
+    // slots.  Never more workgroups than items.
+    const int grid = (int)std::max<long>(1, std::min<long>(n, (long)s->n_cu * s->prm.wgs_per_cu));
     if (far) {
         if (ff_kernel() == PYPO_FF_MATRIX) {
             if (s->prm.tpl == 4) hipLaunchKernelGGL(k_sweep_ff_mfma16, dim3(grid), dim3(TILE), 0, st, ins, items, n);
@@ -463,8 +467,8 @@ int sweep_plan(Sweep *s, SweepPlanParams prm, bool use_graph, hipStream_t st) {
     if (prm.tpl != 1 && prm.tpl != 2 && prm.tpl != 4) prm.tpl = 2;
     if (prm.tpl == 4 && ff_kernel() != PYPO_FF_MATRIX) prm.tpl = 2;
     if (prm.chunk_tiles < 1) prm.chunk_tiles = 1 << 20;
-    if (prm.wgs_per_cu < 1) prm.wgs_per_cu = 2;
-    PYPO_REQUIRE(prm.wgs_per_cu <= 8, ERR_ARG, "sweep: wgs_per_cu must be 1..8");
+    if (prm.wgs_per_cu < 1) prm.wgs_per_cu = 64;
+    PYPO_REQUIRE(prm.wgs_per_cu <= 4096, ERR_ARG, "sweep: wgs_per_cu must be 1..4096");
     PYPO_HIP_TRY(hipStreamSynchronize(st));
     PYPO_HIP_TRY(hipDeviceSynchronize());   // no replay of the old plan may still run
     sweep_free_plan(s);

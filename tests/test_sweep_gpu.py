@@ -50,7 +50,8 @@ def test_sweep_matches_single_propagations(small):
         assert group_err(flat(s.read(i)), D.tensor_to_complex6(e)) < 1e-6
 
 
-@pytest.mark.parametrize("tpl,chunk,wgs", [(1, 1, 1), (2, 1, 3), (1, 4096, 2), (4, 2, 1), (4, 4096, 1)])
+@pytest.mark.parametrize("tpl,chunk,wgs", [(1, 1, 1), (2, 1, 3), (1, 4096, 2), (4, 2, 1), (4, 4096, 1),
+                                          (2, 4096, 64), (2, 1, 4096), (4, 3, 64)])
 def test_launch_shapes_agree(small, tpl, chunk, wgs):
     s, _ = small
     s.plan(tpl=2, chunk_tiles=2, wgs_per_cu=2, use_graph=False)
