@@ -196,6 +196,16 @@ const char *pypo_hip_version(void);
 /* Number of visible HIP devices (0 when none).  Does not create a context. */
 int pypo_hip_device_count(void);
 
+/* Scratch buffers of the device-resident and drop-in calls: each stream that
+ * calls the library keeps its own grow-only set (calls on different streams
+ * are not ordered against each other).  Returns 1 when a set grows with the
+ * stream-ordered allocator (old buffer freed, new one allocated in the
+ * stream's order, nothing else waits), 0 when growth waits for the set's
+ * stream, negative on error (no device).  Growth inside a stream capture is
+ * an error: run a call once before capturing it.  No counterpart in the
+ * reference (its CPU library allocates per call). */
+int pypo_hip_stream_ordered_scratch(void);
+
 /* Precision used by the drop-in PO entry points (default PYPO_PREC_MIXED).
  * Returns the previous value. */
 int pypo_hip_set_precision(int precision);
@@ -483,6 +493,28 @@ void callKernelf_FF(c2Bundlef *res, reflparamsf source, reflparamsf target, refl
 void callKernelf_scalar(arrC1f *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
                         reflcontainerf *ct, arrC1f *inp, float k, float epsilon, float t_direction,
                         int nBlocks, int nThreads);
+/* InterfaceCPU.h:64-101 (InterfaceCPU.cpp:265-455): the reference CPU library's
+ * FP32 interface, for C callers of that library.  Same FP32 path as
+ * callKernelf_* (grids generated into cs / ct, mixed-precision sums, FP32
+ * results); numThreads is the CPU thread count there and is ignored here. */
+void propagateToGridf_JM(c2Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                         reflcontainerf *ct, c2Bundlef *currents, float k, int numThreads, float epsilon,
+                         float t_direction);
+void propagateToGridf_EH(c2Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                         reflcontainerf *ct, c2Bundlef *currents, float k, int numThreads, float epsilon,
+                         float t_direction);
+void propagateToGridf_JMEH(c4Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                           reflcontainerf *ct, c2Bundlef *currents, float k, int numThreads, float epsilon,
+                           float t_direction);
+void propagateToGridf_EHP(c2rBundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                          reflcontainerf *ct, c2Bundlef *currents, float k, int numThreads, float epsilon,
+                          float t_direction);
+void propagateToGridf_scalar(arrC1f *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                             reflcontainerf *ct, arrC1f *field, float k, int numThreads, float epsilon,
+                             float t_direction);
+void propagateToFarFieldf(c2Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                          reflcontainerf *ct, c2Bundlef *currents, float k, int numThreads, float epsilon,
+                          float t_direction);
 /* InterfaceCUDA.h:66-68: ray trace (epsilon = Newton tolerance). */
 void callRTKernel(reflparamsf ctp, cframef *fr_in, cframef *fr_out, float epsilon, float t0, int nBlocks,
                   int nThreads);

@@ -78,6 +78,8 @@ def loadHIPlib():
         lib.pypo_hip_version.restype = ctypes.c_char_p
         lib.pypo_hip_device_count.argtypes = []
         lib.pypo_hip_device_count.restype = i
+        lib.pypo_hip_stream_ordered_scratch.argtypes = []
+        lib.pypo_hip_stream_ordered_scratch.restype = i
         lib.pypo_hip_set_precision.argtypes = [i]
         lib.pypo_hip_set_precision.restype = i
         lib.pypo_hip_set_rt_max_iter.argtypes = [i]

@@ -22,6 +22,10 @@ _fp = ctypes.POINTER(ctypes.c_float)
 _MODES = {"JM": ("callKernelf_JM", S.c2Bundlef), "EH": ("callKernelf_EH", S.c2Bundlef),
           "JMEH": ("callKernelf_JMEH", S.c4Bundlef), "EHP": ("callKernelf_EHP", S.c2rBundlef),
           "FF": ("callKernelf_FF", S.c2Bundlef), "scalar": ("callKernelf_scalar", S.arrC1f)}
+# The reference CPU library's FP32 names (InterfaceCPU.h:64-101): same
+# arguments with numThreads after k instead of nBlocks, nThreads at the end.
+_CPU_NAMES = {"JM": "propagateToGridf_JM", "EH": "propagateToGridf_EH", "JMEH": "propagateToGridf_JMEH",
+              "EHP": "propagateToGridf_EHP", "FF": "propagateToFarFieldf", "scalar": "propagateToGridf_scalar"}
 
 
 def loadGPUlib():
@@ -35,6 +39,10 @@ def loadGPUlib():
         fn = getattr(lib, sym)
         fn.argtypes = [P(bundle), S.reflparamsf, S.reflparamsf, P(S.reflcontainerf), P(S.reflcontainerf),
                        P(src), f, f, f, i, i]
+        fn.restype = None
+        fn = getattr(lib, _CPU_NAMES[mode])
+        fn.argtypes = [P(bundle), S.reflparamsf, S.reflparamsf, P(S.reflcontainerf), P(S.reflcontainerf),
+                       P(src), f, i, f, f]
         fn.restype = None
     lib.callRTKernel.argtypes = [S.reflparamsf, P(S.cframef), P(S.cframef), f, f, i, i]
     lib.callRTKernel.restype = None
@@ -96,8 +104,12 @@ def _complex_rows(obj):
     return rows
 
 
-def PyPO_GPUf(source, target, runPODict):
-    """PO propagation through the FP32 entry points; result types as BindGPU.PyPO_GPUf."""
+def PyPO_GPUf(source, target, runPODict, interface="gpu"):
+    """PO propagation through the FP32 entry points; result types as BindGPU.PyPO_GPUf.
+    interface "gpu" calls callKernelf_* (InterfaceCUDA.h), "cpu" the CPU
+    library's propagateToGridf_* / propagateToFarFieldf (InterfaceCPU.h)."""
+    if interface not in ("gpu", "cpu"):
+        raise ValueError(f"interface must be 'gpu' or 'cpu', not {interface!r}")
     lib = loadGPUlib()
     mode = runPODict["mode"]
     if mode not in _MODES:
@@ -116,9 +128,15 @@ def PyPO_GPUf(source, target, runPODict):
     res = _alloc(bundle, _names(bundle), gt)
     tdir = -1.0 if runPODict.get("exp", "fwd") == "fwd" else 1.0
     nthreads = int(runPODict.get("nThreads", 256))
-    getattr(lib, sym)(ctypes.byref(res), fill_reflparamsf(source), fill_reflparamsf(target), ctypes.byref(cs),
-                      ctypes.byref(ct), ctypes.byref(src), float(runPODict["k"]),
-                      float(runPODict.get("epsilon", 1.0)), tdir, (gt + nthreads - 1) // nthreads, nthreads)
+    if interface == "gpu":
+        getattr(lib, sym)(ctypes.byref(res), fill_reflparamsf(source), fill_reflparamsf(target), ctypes.byref(cs),
+                          ctypes.byref(ct), ctypes.byref(src), float(runPODict["k"]),
+                          float(runPODict.get("epsilon", 1.0)), tdir, (gt + nthreads - 1) // nthreads, nthreads)
+    else:
+        sym = _CPU_NAMES[mode]
+        getattr(lib, sym)(ctypes.byref(res), fill_reflparamsf(source), fill_reflparamsf(target), ctypes.byref(cs),
+                          ctypes.byref(ct), ctypes.byref(src), float(runPODict["k"]),
+                          int(runPODict.get("nThreads", 1)), float(runPODict.get("epsilon", 1.0)), tdir)
     B.check(lib, sym)
     a = [res._keep[n].astype(np.float64).reshape(shape) for n in _names(bundle)]
 

@@ -4,6 +4,7 @@
 // pool, library stream, error channel, host-side random deviates.
 
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <random>
 #include <vector>
@@ -42,13 +43,24 @@ namespace pypo {
 // ---- library state ------------------------------------------------------------
 namespace {
 
-std::recursive_mutex g_mu;         // one library call at a time (shared pool)
-void *g_pool[SLOT_COUNT] = {};
-size_t g_pool_bytes[SLOT_COUNT] = {};
+std::recursive_mutex g_mu;         // one library call at a time on the host
+
+// Scratch buffers of the calls enqueued on one stream.  Calls on one stream
+// run in that stream's order and share its set; calls on different streams
+// use different sets and need no ordering against each other, so
+// device-resident work on several streams overlaps.
+struct PoolSet {
+    void *buf[SLOT_COUNT] = {};
+    size_t bytes[SLOT_COUNT] = {};
+    hipEvent_t done = nullptr;     // completion of the last call that used the set
+};
+std::map<hipStream_t, PoolSet> g_sets;   // nodes never move: pointers below stay valid
+PoolSet *g_cur = nullptr;          // set of the call being enqueued
+hipStream_t g_cur_st = nullptr;
+PoolSet *g_open = nullptr;         // set of a call that stopped before end_call
+hipStream_t g_open_st = nullptr;
+bool g_async_alloc = false;        // stream-ordered allocation (hipMallocAsync) available
 hipStream_t g_stream = nullptr;
-hipEvent_t g_done = nullptr;        // completion of the last enqueued call
-hipStream_t g_open = nullptr;       // stream of a call that stopped before end_call
-bool g_open_valid = false;
 int g_fail_allocs = 0;              // fault injection: pool requests left to fail
 int g_precision = PYPO_PREC_MIXED;
 int g_rt_max_iter = 50;
@@ -62,29 +74,39 @@ int ensure_init() {
         set_error(ERR_NODEV, "no HIP device visible");
         return ERR_NODEV;
     }
+    int dev = 0, pools = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, dev) == hipSuccess)
+        g_async_alloc = pools != 0;
     PYPO_HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-    PYPO_HIP_TRY(hipEventCreateWithFlags(&g_done, hipEventDisableTiming));
     if (const char *v = std::getenv("PYPO_HIP_FAIL_ALLOCS")) g_fail_allocs = std::max(0, std::atoi(v));
     return 0;
 }
 
-// Order this call after every earlier call (they share pooled buffers).  A
-// call that stopped on an error after enqueuing work never reached end_call:
-// its stream is marked now, so this call also waits for that work.
+// Start a call on stream st: select st's buffer set.  A call that stopped on
+// an error after enqueuing work never reached end_call; its set's completion
+// is marked on its stream now.  Waiting on the set's last completion orders
+// this call after every earlier user of the set, which for the same stream is
+// already the case (it matters when a new stream reuses a destroyed stream's
+// handle).
 int begin_call(hipStream_t st) {
-    if (g_open_valid) {
-        (void)hipEventRecord(g_done, g_open);
-        g_open_valid = false;
+    if (g_open) {
+        (void)hipEventRecord(g_open->done, g_open_st);
+        g_open = nullptr;
     }
-    if (g_done) PYPO_HIP_TRY(hipStreamWaitEvent(st, g_done, 0));
-    g_open = st;
-    g_open_valid = true;
+    PoolSet &ps = g_sets[st];
+    if (!ps.done) PYPO_HIP_TRY(hipEventCreateWithFlags(&ps.done, hipEventDisableTiming));
+    else PYPO_HIP_TRY(hipStreamWaitEvent(st, ps.done, 0));
+    g_cur = &ps;
+    g_cur_st = st;
+    g_open = &ps;
+    g_open_st = st;
     return 0;
 }
 
 int end_call(hipStream_t st) {
-    PYPO_HIP_TRY(hipEventRecord(g_done, st));
-    g_open_valid = false;
+    PYPO_HIP_TRY(hipEventRecord(g_cur->done, st));
+    g_open = nullptr;
     return 0;
 }
 
@@ -105,26 +127,46 @@ void *pool_get(int slot, size_t bytes, int *err) {
         *err = ERR_ALLOC;
         return nullptr;
     }
+    PoolSet &ps = *g_cur;
     if (bytes == 0) bytes = 256;
-    if (g_pool_bytes[slot] >= bytes) return g_pool[slot];
-    if (g_pool[slot]) {
-        // earlier asynchronous calls may still read the old buffer
-        (void)hipDeviceSynchronize();
-        (void)hipFree(g_pool[slot]);
-        g_pool[slot] = nullptr;
-        g_pool_bytes[slot] = 0;
+    if (ps.bytes[slot] >= bytes) return ps.buf[slot];
+    const size_t want = bytes + bytes / 8;   // headroom for slightly larger calls
+    hipError_t e;
+    if (g_async_alloc) {
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(g_cur_st, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone) {
+            set_error(ERR_ARG, "a scratch buffer would have to grow inside a stream capture: run the "
+                               "same call once on this stream before capturing it");
+            *err = ERR_ARG;
+            return nullptr;
+        }
+        // Stream order frees the old buffer after the earlier calls on this
+        // set are done with it and makes the new one ready for what follows;
+        // nothing else on the device waits.
+        if (ps.buf[slot]) (void)hipFreeAsync(ps.buf[slot], g_cur_st);
+        ps.buf[slot] = nullptr;
+        ps.bytes[slot] = 0;
+        e = hipMallocAsync(&ps.buf[slot], want, g_cur_st);
+    } else {
+        if (ps.buf[slot]) {
+            // earlier calls on this set may still read the old buffer
+            (void)hipEventSynchronize(ps.done);
+            (void)hipStreamSynchronize(g_cur_st);
+            (void)hipFree(ps.buf[slot]);
+        }
+        ps.buf[slot] = nullptr;
+        ps.bytes[slot] = 0;
+        e = hipMalloc(&ps.buf[slot], want);
     }
-    size_t want = bytes + bytes / 8;   // headroom for slightly larger calls
-    hipError_t e = hipMalloc(&g_pool[slot], want);
     if (e != hipSuccess) {
-        g_pool[slot] = nullptr;
-        set_error(ERR_ALLOC, std::string("hipMalloc of ") + std::to_string(want) +
-                                 " bytes failed: " + hipGetErrorString(e));
+        ps.buf[slot] = nullptr;
+        set_error(ERR_ALLOC, std::string("allocation of ") + std::to_string(want) +
+                                 " device bytes failed: " + hipGetErrorString(e));
         *err = ERR_ALLOC;
         return nullptr;
     }
-    g_pool_bytes[slot] = want;
-    return g_pool[slot];
+    ps.bytes[slot] = want;
+    return ps.buf[slot];
 }
 
 namespace {
@@ -430,6 +472,13 @@ int pypo_hip_device_count(void) {
     return n;
 }
 
+int pypo_hip_stream_ordered_scratch(void) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return -err;
+    return g_async_alloc ? 1 : 0;
+}
+
 int pypo_hip_set_precision(int precision) {
     std::lock_guard<std::recursive_mutex> lock(g_mu);
     int old = g_precision;
@@ -661,6 +710,41 @@ void callKernelf_FF(c2Bundlef *res, reflparamsf source, reflparamsf target, refl
 void callKernelf_scalar(arrC1f *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
                         reflcontainerf *ct, arrC1f *inp, float k, float epsilon, float t_direction, int, int) {
     po_float(PYPO_MODE_SCALAR, res, source, target, cs, ct, inp, k, epsilon, t_direction);
+}
+
+// InterfaceCPU.h:64-101: the CPU library's FP32 names, argument order with
+// numThreads after k; the thread count has no meaning on the GPU.
+void propagateToGridf_JM(c2Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                         reflcontainerf *ct, c2Bundlef *currents, float k, int, float epsilon, float t_direction) {
+    po_float(PYPO_MODE_JM, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+void propagateToGridf_EH(c2Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                         reflcontainerf *ct, c2Bundlef *currents, float k, int, float epsilon, float t_direction) {
+    po_float(PYPO_MODE_EH, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+void propagateToGridf_JMEH(c4Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                           reflcontainerf *ct, c2Bundlef *currents, float k, int, float epsilon,
+                           float t_direction) {
+    po_float(PYPO_MODE_JMEH, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+void propagateToGridf_EHP(c2rBundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                          reflcontainerf *ct, c2Bundlef *currents, float k, int, float epsilon,
+                          float t_direction) {
+    po_float(PYPO_MODE_EHP, res, source, target, cs, ct, currents, k, epsilon, t_direction);
+}
+
+void propagateToGridf_scalar(arrC1f *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                             reflcontainerf *ct, arrC1f *field, float k, int, float epsilon, float t_direction) {
+    po_float(PYPO_MODE_SCALAR, res, source, target, cs, ct, field, k, epsilon, t_direction);
+}
+
+void propagateToFarFieldf(c2Bundlef *res, reflparamsf source, reflparamsf target, reflcontainerf *cs,
+                          reflcontainerf *ct, c2Bundlef *currents, float k, int, float epsilon,
+                          float t_direction) {
+    po_float(PYPO_MODE_FF, res, source, target, cs, ct, currents, k, epsilon, t_direction);
 }
 
 void callRTKernel(reflparamsf ctp, cframef *fr_in, cframef *fr_out, float epsilon, float t0, int, int) {

@@ -32,6 +32,12 @@ def test_header_declares_the_reference_entry_points():
                      "propagateToGrid_EHP", "propagateToGrid_scalar", "propagateToFarField",
                      "propagateRays", "generateGrid"):
         assert ref_name in names
+    # InterfaceCPU.h:64-101 FP32 set and InterfaceCUDA.h:29-69
+    for ref_name in ("propagateToGridf_JM", "propagateToGridf_EH", "propagateToGridf_JMEH",
+                     "propagateToGridf_EHP", "propagateToGridf_scalar", "propagateToFarFieldf",
+                     "callKernelf_JM", "callKernelf_EH", "callKernelf_JMEH", "callKernelf_EHP",
+                     "callKernelf_FF", "callKernelf_scalar", "callRTKernel"):
+        assert ref_name in names
 
 
 def test_library_exports_every_declared_symbol():
@@ -152,3 +158,8 @@ def test_new_entry_points_report_errors_without_a_device():
     fr = T.frame(2, np.zeros(2), np.zeros(2), np.zeros(2), np.zeros(2), np.zeros(2), np.ones(2))
     with pytest.raises(B.PyPOHIPError):
         BG.RT_GPUf({"fr_in": fr, "t_name": src, "tol": 1e-6, "t0": 1.0})
+    with pytest.raises(B.PyPOHIPError):
+        BG.PyPO_GPUf(src, src, {"mode": "FF", "k": 1.0, "s_current": T.currents(z, z, z, z, z, z)},
+                     interface="cpu")
+    assert lib.pypo_hip_stream_ordered_scratch() < 0
+    lib.pypo_hip_last_error(buf, 256)

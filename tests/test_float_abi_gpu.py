@@ -1,6 +1,7 @@
 """The reference GPU library's single-precision entry points (callKernelf_*,
 callRTKernel; src/CUDA/InterfaceCUDA.h:29-69) through pypo_amd.bind_gpu, the
-twin of the reference's BindGPU.
+twin of the reference's BindGPU, and the CPU library's FP32 names
+(propagateToGridf_*, propagateToFarFieldf; src/CPU/InterfaceCPU.h:64-101).
 
 The inputs cross the boundary in FP32, so the checker runs the FP64 CPU
 restatement on the same FP32-rounded inputs (surface parameters, currents,
@@ -58,6 +59,29 @@ def test_callKernelf_matches_checker_on_fp32_inputs(hip, port, name):
     print(f"{name} fp32 entry point: {err:.2e}")
     assert err <= 1e-5
     assert type(got) is type(ref) or (isinstance(got, list) and [type(x) for x in got] == [type(x) for x in ref])
+
+
+@pytest.mark.parametrize("name", sorted(C.po_cases()))
+def test_cpu_library_fp32_names_match_callKernelf(hip, name):
+    """propagateToGridf_* / propagateToFarFieldf (InterfaceCPU.h:64-101) run the
+    same FP32 path as callKernelf_*: bit-identical results, whatever numThreads says."""
+    c = C.po_cases()[name]
+    src, tgt = _f32_refl(c["source"]), _f32_refl(c["target"])
+    shape = gridsize(c["source"])
+    k = float(np.float32(2 * np.pi / c["lam"]))
+    eps = float(np.float32(c.get("epsilon", 1.0)))
+    if c["mode"] == "scalar":
+        d = {"mode": "scalar", "k": k, "exp": c["exp"], "epsilon": eps,
+             "s_scalarfield": T.scalarfield(_f32_complex(C.random_scalar(shape, c["seed"]).S))}
+    else:
+        cur = C.random_currents(shape, c["seed"])
+        d = {"mode": c["mode"], "k": k, "exp": c["exp"], "epsilon": eps,
+             "s_current": T.currents(*[_f32_complex(x) for x in cur.components()])}
+    a = flat(BG.PyPO_GPUf(src, tgt, d, interface="gpu"))
+    b = flat(BG.PyPO_GPUf(src, tgt, dict(d, nThreads=7), interface="cpu"))
+    assert len(a) == len(b)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
 
 
 def test_callRTKernel_matches_checker_on_fp32_inputs(hip, port):

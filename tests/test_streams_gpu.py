@@ -1,0 +1,73 @@
+"""Device-resident calls on several streams.  Each stream that calls the
+library has its own grow-only scratch set, so calls on different streams are
+not ordered against each other, and a set that must grow frees and allocates
+in its stream's order (no device-wide synchronisation).  Results must be
+bit-identical to the same calls made one after another on one stream."""
+
+import numpy as np
+import pytest
+
+from pypo_amd import geometry as G
+from pypo_amd import types as T
+
+pytestmark = pytest.mark.gpu
+
+
+def _surfaces(n_src, n_tgt):
+    src = G.plane({"gmode": "xy", "lims_x": [-5, 5], "lims_y": [-5, 5], "gridsize": [n_src, n_src + 2]})
+    tgt = G.plane({"gmode": "xy", "lims_x": [-6, 6], "lims_y": [-6, 6], "gridsize": [n_tgt, n_tgt + 1]})
+    tgt["transf"] = G.mat_translate([0, 0, 40.0])
+    rng = np.random.default_rng(n_src)
+    shape = (n_src, n_src + 2)
+    cur = T.currents(*[rng.normal(size=shape) + 1j * rng.normal(size=shape) for _ in range(6)])
+    return src, tgt, cur
+
+
+def _run(D, case, mode, prec):
+    src, tgt, cur = case
+    gs, gt = D.generate_grid(src), D.generate_grid(tgt)
+    return D.propagate(mode, gs, gt, D.currents_tensor(cur), src, 2 * np.pi / 2.0, precision=prec)
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_calls_on_two_streams_match_one_stream(hip, prec):
+    import torch
+    from pypo_amd import device as D
+    cases = [_surfaces(31, 41), _surfaces(57, 23), _surfaces(19, 77)]
+    modes = ["EH", "JM", "FF"]
+    ref = [_run(D, c, m, prec).clone() for c, m in zip(cases, modes)]
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    outs = []
+    for rep in range(3):             # interleaved, no host synchronisation between calls
+        for i, (c, m) in enumerate(zip(cases, modes)):
+            with torch.cuda.stream(s1 if (i + rep) % 2 == 0 else s2):
+                outs.append((i, _run(D, c, m, prec)))
+    torch.cuda.synchronize()
+    for i, o in outs:
+        assert torch.equal(o, ref[i]), (i, modes[i])
+
+
+def test_scratch_grows_with_the_stream_ordered_allocator(hip):
+    assert hip.pypo_hip_stream_ordered_scratch() == 1
+
+
+def test_scratch_growth_in_stream_order(hip):
+    """small -> large -> small on a fresh stream without host synchronisation:
+    the large call grows the stream's buffers while the small call's work may
+    still be queued, and the last call reuses the grown buffers."""
+    import torch
+    from pypo_amd import device as D
+    small, large = _surfaces(23, 29), _surfaces(151, 97)
+    ref_s = _run(D, small, "EH", "mixed").clone()
+    ref_l = _run(D, large, "EH", "mixed").clone()
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        a = _run(D, small, "EH", "mixed")
+        b = _run(D, large, "EH", "mixed")
+        c = _run(D, small, "EH", "mixed")
+    s.synchronize()
+    assert torch.equal(a, ref_s)
+    assert torch.equal(b, ref_l)
+    assert torch.equal(c, ref_s)
