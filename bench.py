@@ -215,7 +215,7 @@ def run_system_chain(torch, w0, feed, dish, ff):
 def run_sweep(torch, log):
     """Batched sweep: 64 frequencies (80-120 GHz) x 4 feed offsets = 256
     JM -> FF instances, dish grids 201^2 .. 401^2 (uneven), one 181^2 far
-    field, each stage a persistent launch, the run a replayed hipGraph; launch
+    field, each sum stage one launch, the run a replayed hipGraph; launch
     shape from the autotuner (fresh tuning, no cache)."""
     from pypo_amd import autotune as AT
     from pypo_amd import sweep as SW
@@ -243,7 +243,7 @@ def run_sweep(torch, log):
     log(f"sweep {len(feeds)} instances: {dt * 1e3:.1f} ms/run ({pairs / dt:.3e} pairs/s), tuned {best} "
         f"in {tune_s:.1f} s")
     return {"instances": len(feeds), "pairs_per_run": pairs, "ms_per_run": round(dt * 1e3, 2),
-            "pairs_per_s": pairs / dt, "launch": "hipGraph replay of 2 persistent stages",
+            "pairs_per_s": pairs / dt, "launch": "hipGraph replay: JM sum, JM finish, pack, FF sum, FF finish",
             "tuned": AT.describe(best), "tuning_s": round(tune_s, 1), "setup_s": round(setup_s, 1),
             "slowest_candidate_ms": round(worst["seconds"] * 1e3, 1),
             "tuning_table": table_ms, "tuning_timing": "median of 3 synchronised replays per candidate",

@@ -268,8 +268,9 @@ int pypo_hip_trace_dev(const reflparams *surfs, int n_surf, cframe *frames,
  * uneven grid sizes sharing one far-field grid.  Per instance: feed surface,
  * dish surface, wavenumber k (1/mm) and the feed currents (host arrays of the
  * feed grid's size).  The JM step (feed -> dish, PEC currents) and the FF
- * step (dish -> ff) of every instance run as one persistent launch each over
- * a static work list sorted by cost; mixed precision.  tpl (targets per lane:
+ * step (dish -> ff) of every instance run as one launch each over a static
+ * work list sorted by cost (a run: these two, a finish kernel after each and
+ * one packing kernel between them); mixed precision.  tpl (targets per lane:
  * 1 or 2, or 4 = 16 M-blocks of the matrix-core far field at one wave per
  * SIMD), chunk_tiles (256-source tiles per work item) and wgs_per_cu
  * (workgroups per CU in each launch, 1..4096, never more than the stage's

@@ -1,4 +1,4 @@
-"""Autotuner for the persistent sweep launches.
+"""Autotuner for the sweep launches.
 
 Times every candidate launch shape on the actual sweep, keeps the fastest,
 and caches the choice with the whole timing table in a JSON file keyed by
@@ -6,7 +6,7 @@ the device and a signature of the sweep (instance count, pair counts).
 Dimensions: targets per lane (TPL: the far field's target block is 256 TPL
 targets per workgroup, its matrix-core kernel runs 4 TPL M-blocks of 16
 targets per wave; TPL 4 is the 16-M-block kernel at one wave per SIMD),
-256-source tiles per work item, and persistent workgroups per CU.  Each
+256-source tiles per work item, and workgroups per CU in each launch.  Each
 candidate is timed as the median of `reps` individually synchronised
 replays after one warm run.  Cache path: $PYPO_AMD_TUNE_CACHE or
 ~/.cache/pypo_amd/autotune.json.

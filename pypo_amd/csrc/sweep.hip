@@ -1,18 +1,19 @@
 // sweep.hip -- batched JM -> FF sweeps (frequency x feed offset, uneven grid
-// sizes) as persistent launches over a static work list.
+// sizes): one launch per stage over a static work list.
 //
 // A sweep holds n_inst instances, each a feed (source of the JM step), a
 // dish (target of JM, source of FF) and a wavenumber; all instances share
-// one far-field grid.  Each stage is one persistent launch: workgroup w
-// processes work items w, w + G, w + 2G, ... of a list sorted by cost
-// (longest first), so the launch is deterministic, needs no atomics or
-// counters, and every workgroup leaves the loop when the list ends.  A
+// one far-field grid.  Each stage is one launch: workgroup w processes work
+// items w, w + G, w + 2G, ... of a list sorted by cost (longest first), so
+// the launch needs no atomics or counters and every workgroup leaves the
+// loop when the list ends.  G may reach one workgroup per item; the
+// dispatcher then starts the items longest first as slots free.  A
 // work item is (instance, 256*TPL-target block, source chunk); its FP64
 // partial sums go to a per-instance slot per chunk and batched finish
 // kernels reduce the chunks in order and apply the epilogues (J = 2 n x H
-// for JM, the far-field projection for FF).  The whole run -- JM sum, JM
-// finish, packing the dish currents, FF sum, FF finish -- can be captured
-// into one hipGraph and replayed.
+// for JM, the far-field projection for FF).  A run is five kernels -- JM
+// sum, JM finish, packing the dish currents, FF sum, FF finish -- that can
+// be captured into one hipGraph and replayed.
 
 #include <algorithm>
 #include <cstring>

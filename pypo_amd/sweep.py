@@ -1,9 +1,10 @@
 """Batched JM -> FF sweeps on the GPU (pypo_hip_sweep_* in include/pypo_hip.h).
 
 A sweep is a set of instances -- feed surface + feed currents, dish surface,
-wavenumber -- of uneven grid sizes sharing one far-field grid.  Each stage
-(JM on all dishes, FF of all dishes) is one persistent launch over a static,
-cost-sorted work list, and the whole run can be captured into a hipGraph.
+wavenumber -- of uneven grid sizes sharing one far-field grid.  Each sum
+stage (JM on all dishes, FF of all dishes) is one launch over a static,
+cost-sorted work list; a run is five kernels (JM sum, JM finish, pack, FF
+sum, FF finish) that can be captured into one hipGraph.
 The reference runs such sweeps as Python loops over System.runPO calls.
 """
 
