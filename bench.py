@@ -343,7 +343,7 @@ def cpu_chain(feed, dish, cur_feed, sub_rows, threads):
     nrow = len(sub_rows)
     sub = G.plane({"gmode": "AoE", "lims_Az": [-FF_LIM, FF_LIM], "lims_El": [-FF_LIM, FF_LIM],
                    "gridsize": [nrow, N_FF]})
-    # median of 3 runs (BASELINE.md's protocol for CPU jobs under a minute)
+    # median of 3 runs: a CPU job under a minute is timed as the median of at least three
     runs = []
     for _ in range(3):
         t = time.perf_counter()
