@@ -47,6 +47,25 @@ struct TileLoader {
 };
 
 // Wave-level reductions (64 lanes) through cross-lane shuffles.
+// Work item (target block tb, source split sp) of a PO sum workgroup, for a
+// grid of target blocks x splits.  The dispatcher deals workgroups out in
+// linear-id order, round-robin over the XCDs (XCD = id % kXcds), so taken
+// as they come every XCD works on the same split at once and its own L2
+// fetches that split's records, again after each eviction.  This bijection
+// hands XCD x a contiguous run of the split-major item list instead: the
+// XCDs work on different splits and the workgroups sharing an XCD's L2 at
+// one time read the same records.  Which workgroup computes which item
+// does not change any result.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void xcd_item(int &tb, int &sp) {
+    const int nx = gridDim.x, n = nx * gridDim.y;
+    const int lin = blockIdx.y * nx + blockIdx.x;
+    const int x = lin % kXcds, q = n / kXcds, r = n % kXcds;
+    const int item = x * q + min(x, r) + lin / kXcds;   // XCD x: items [x q + min(x, r), ...)
+    tb = item % nx;
+    sp = item / nx;
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -127,6 +127,8 @@ __global__ __launch_bounds__(256) void k_ff_mixed(const SrcRec32 *__restrict__ s
     __shared__ float scratch[4];
     __shared__ int follow_ok[4];   // per wave: every follower of its 64 records within the bound
     const int tid = threadIdx.x, lane = tid & 63;
+    int tb, sp;
+    xcd_item(tb, sp);
     double rx[TPL], ry[TPL], rz[TPL];
     float fx[TPL], fy[TPL], fz[TPL];
     Cone cone;
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(256) void k_ff_mixed(const SrcRec32 *__restrict__ s
         bool any = false;
 #pragma unroll
         for (int j = 0; j < TPL; j++) {
-            int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+            int t = tb * (TILE * TPL) + j * TILE + tid;
             double x = 0, y = 0, z = 0;
             if (t < n_tgt) {
                 double st, ct, sp, cp;
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(256) void k_ff_mixed(const SrcRec32 *__restrict__ s
             float d = 1.f;
 #pragma unroll
             for (int j = 1; j < TPL; j++) {
-                int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+                int t = tb * (TILE * TPL) + j * TILE + tid;
                 if (t < n_tgt) d = fminf(d, fx[j] * cone.ax + fy[j] * cone.ay + fz[j] * cone.az);
             }
             cone.cos_half = fminf(cone.cos_half, block_min(d, scratch));
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(256) void k_ff_mixed(const SrcRec32 *__restrict__ s
         ubz = fminf(1.f, fabsf(cone.az) + sn);
     }
 
-    const int s_begin = blockIdx.y * chunk;
+    const int s_begin = sp * chunk;
     const int s_end = min(n_src, s_begin + chunk);
     const uint4 *g = reinterpret_cast<const uint4 *>(src);
     TileLoader<6> ld;
@@ -214,11 +216,11 @@ __global__ __launch_bounds__(256) void k_ff_mixed(const SrcRec32 *__restrict__ s
     }
 #pragma unroll
     for (int j = 0; j < TPL; j++) {
-        int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+        int t = tb * (TILE * TPL) + j * TILE + tid;
         if (t < n_tgt) {
 #pragma unroll
             for (int q = 0; q < 12; q++)
-                partial[((size_t)blockIdx.y * 12 + q) * n_tgt + t] = tot[j][q];
+                partial[((size_t)sp * 12 + q) * n_tgt + t] = tot[j][q];
         }
     }
 }
@@ -263,6 +265,8 @@ __global__ __launch_bounds__(256) void k_ff_fp64(const SrcRec64 *__restrict__ sr
     __shared__ SrcRec64 tile[TILE];
     __shared__ float scratch[4];
     const int tid = threadIdx.x;
+    int tb, sp;
+    xcd_item(tb, sp);
     double ux[TPL], uy[TPL], uz[TPL];
     double rx[TPL], ry[TPL], rz[TPL];
     Cone cone;
@@ -270,7 +274,7 @@ __global__ __launch_bounds__(256) void k_ff_fp64(const SrcRec64 *__restrict__ sr
         bool any = false;
 #pragma unroll
         for (int j = 0; j < TPL; j++) {
-            int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+            int t = tb * (TILE * TPL) + j * TILE + tid;
             double x = 0, y = 0, z = 0;
             if (t < n_tgt) {
                 double st, ct, sp, cp;
@@ -287,7 +291,7 @@ __global__ __launch_bounds__(256) void k_ff_fp64(const SrcRec64 *__restrict__ sr
             float d = 1.f;
 #pragma unroll
             for (int j = 1; j < TPL; j++) {
-                int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+                int t = tb * (TILE * TPL) + j * TILE + tid;
                 if (t < n_tgt) d = fminf(d, (float)(ux[j] * cone.ax + uy[j] * cone.ay + uz[j] * cone.az));
             }
             cone.cos_half = fminf(cone.cos_half, block_min(d, scratch));
@@ -299,7 +303,7 @@ __global__ __launch_bounds__(256) void k_ff_fp64(const SrcRec64 *__restrict__ sr
 #pragma unroll
         for (int q = 0; q < 12; q++) acc[j][q] = 0.0;
 
-    const int s_begin = blockIdx.y * chunk;
+    const int s_begin = sp * chunk;
     const int s_end = min(n_src, s_begin + chunk);
     const uint4 *g = reinterpret_cast<const uint4 *>(src);
     TileLoader<9> ld;
@@ -321,11 +325,11 @@ __global__ __launch_bounds__(256) void k_ff_fp64(const SrcRec64 *__restrict__ sr
     }
 #pragma unroll
     for (int j = 0; j < TPL; j++) {
-        int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+        int t = tb * (TILE * TPL) + j * TILE + tid;
         if (t < n_tgt) {
 #pragma unroll
             for (int q = 0; q < 12; q++)
-                partial[((size_t)blockIdx.y * 12 + q) * n_tgt + t] = acc[j][q];
+                partial[((size_t)sp * 12 + q) * n_tgt + t] = acc[j][q];
         }
     }
 }

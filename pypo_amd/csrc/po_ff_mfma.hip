@@ -14,18 +14,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const SrcRec32 *__restrict__ src, const TileMeta *__restrict__ meta, int n_src, int chunk,
     const double *__restrict__ th, const double *__restrict__ ph, int n_tgt, double rev_scale,
     double *__restrict__ partial) {
-    const int s0 = blockIdx.y * chunk;
-    mfma::ff_mfma_body<4>(src, meta, s0, min(n_src, s0 + chunk), blockIdx.x * 256, th, ph, n_tgt, rev_scale,
-                          partial + (size_t)blockIdx.y * 12 * n_tgt);
+    int tb, sp;
+    xcd_item(tb, sp);
+    const int s0 = sp * chunk;
+    mfma::ff_mfma_body<4>(src, meta, s0, min(n_src, s0 + chunk), tb * 256, th, ph, n_tgt, rev_scale,
+                          partial + (size_t)sp * 12 * n_tgt);
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_ff_mfma8(
     const SrcRec32 *__restrict__ src, const TileMeta *__restrict__ meta, int n_src, int chunk,
     const double *__restrict__ th, const double *__restrict__ ph, int n_tgt, double rev_scale,
     double *__restrict__ partial) {
-    const int s0 = blockIdx.y * chunk;
-    mfma::ff_mfma_body<8>(src, meta, s0, min(n_src, s0 + chunk), blockIdx.x * 512, th, ph, n_tgt, rev_scale,
-                          partial + (size_t)blockIdx.y * 12 * n_tgt);
+    int tb, sp;
+    xcd_item(tb, sp);
+    const int s0 = sp * chunk;
+    mfma::ff_mfma_body<8>(src, meta, s0, min(n_src, s0 + chunk), tb * 512, th, ph, n_tgt, rev_scale,
+                          partial + (size_t)sp * 12 * n_tgt);
 }
 
 }  // namespace

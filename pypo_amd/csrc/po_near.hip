@@ -29,13 +29,15 @@ __global__ __launch_bounds__(256) void k_near_mixed(const SrcRec32 *__restrict__
                                                     double *__restrict__ partial) {
     __shared__ SrcRec32 tile[TILE];
     const int tid = threadIdx.x;
+    int tb, sp;
+    xcd_item(tb, sp);
     const double rev_scale = tdir * k / kTwoPi;   // exp(j t k R)
     const float kinv = (float)(1.0 / k);
     const float tf = (float)tdir;
     double px[TPL], py[TPL], pz[TPL];
 #pragma unroll
     for (int j = 0; j < TPL; j++) {
-        int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+        int t = tb * (TILE * TPL) + j * TILE + tid;
         // padding lanes sit far away so that every distance stays finite
         px[j] = (t < n_tgt) ? tx[t] : 1.0e6;
         py[j] = (t < n_tgt) ? ty[t] : 1.0e6;
@@ -48,7 +50,7 @@ __global__ __launch_bounds__(256) void k_near_mixed(const SrcRec32 *__restrict__
 #pragma unroll
         for (int q = 0; q < 12; q++) { acc[j][q] = 0.f; tot[j][q] = 0.0; }
 
-    const int s_begin = blockIdx.y * chunk;
+    const int s_begin = sp * chunk;
     const int s_end = min(n_src, s_begin + chunk);
     const uint4 *g = reinterpret_cast<const uint4 *>(src);
     TileLoader<6> ld;
@@ -70,11 +72,11 @@ __global__ __launch_bounds__(256) void k_near_mixed(const SrcRec32 *__restrict__
     }
 #pragma unroll
     for (int j = 0; j < TPL; j++) {
-        int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+        int t = tb * (TILE * TPL) + j * TILE + tid;
         if (t < n_tgt) {
 #pragma unroll
             for (int q = WANT_E ? 0 : 6; q < 12; q++)
-                partial[((size_t)blockIdx.y * 12 + q) * n_tgt + t] = tot[j][q];
+                partial[((size_t)sp * 12 + q) * n_tgt + t] = tot[j][q];
         }
     }
 }
@@ -164,11 +166,13 @@ __global__ __launch_bounds__(256) void k_near_fp64(const SrcRec64 *__restrict__ 
                                                    double *__restrict__ partial) {
     __shared__ SrcRec64 tile[TILE];
     const int tid = threadIdx.x;
+    int tb, sp;
+    xcd_item(tb, sp);
     const double rev_scale = tdir * k / kTwoPi;
     double px[TPL], py[TPL], pz[TPL];
 #pragma unroll
     for (int j = 0; j < TPL; j++) {
-        int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+        int t = tb * (TILE * TPL) + j * TILE + tid;
         px[j] = (t < n_tgt) ? tx[t] : 1.0e6;
         py[j] = (t < n_tgt) ? ty[t] : 1.0e6;
         pz[j] = (t < n_tgt) ? tz[t] : 1.0e6;
@@ -179,7 +183,7 @@ __global__ __launch_bounds__(256) void k_near_fp64(const SrcRec64 *__restrict__ 
 #pragma unroll
         for (int q = 0; q < 12; q++) acc[j][q] = 0.0;
 
-    const int s_begin = blockIdx.y * chunk;
+    const int s_begin = sp * chunk;
     const int s_end = min(n_src, s_begin + chunk);
     const uint4 *g = reinterpret_cast<const uint4 *>(src);
     TileLoader<9> ld;
@@ -197,11 +201,11 @@ __global__ __launch_bounds__(256) void k_near_fp64(const SrcRec64 *__restrict__ 
     }
 #pragma unroll
     for (int j = 0; j < TPL; j++) {
-        int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+        int t = tb * (TILE * TPL) + j * TILE + tid;
         if (t < n_tgt) {
 #pragma unroll
             for (int q = 0; q < 12; q++)
-                partial[((size_t)blockIdx.y * 12 + q) * n_tgt + t] = acc[j][q];
+                partial[((size_t)sp * 12 + q) * n_tgt + t] = acc[j][q];
         }
     }
 }

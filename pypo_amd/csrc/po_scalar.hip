@@ -16,10 +16,12 @@ __global__ __launch_bounds__(256) void k_scalar_mixed(const SrcRecS32 *__restric
                                                       double rev_scale, double *__restrict__ partial) {
     __shared__ SrcRecS32 tile[TILE];
     const int tid = threadIdx.x;
+    int tb, sp;
+    xcd_item(tb, sp);
     double px[TPL], py[TPL], pz[TPL];
 #pragma unroll
     for (int j = 0; j < TPL; j++) {
-        int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+        int t = tb * (TILE * TPL) + j * TILE + tid;
         px[j] = (t < n_tgt) ? tx[t] : 1.0e6;
         py[j] = (t < n_tgt) ? ty[t] : 1.0e6;
         pz[j] = (t < n_tgt) ? tz[t] : 1.0e6;
@@ -28,7 +30,7 @@ __global__ __launch_bounds__(256) void k_scalar_mixed(const SrcRecS32 *__restric
     double tr[TPL], ti[TPL];
 #pragma unroll
     for (int j = 0; j < TPL; j++) { ar[j] = ai[j] = 0.f; tr[j] = ti[j] = 0.0; }
-    const int s_begin = blockIdx.y * chunk;
+    const int s_begin = sp * chunk;
     const int s_end = min(n_src, s_begin + chunk);
     const uint4 *g = reinterpret_cast<const uint4 *>(src);
     TileLoader<2> ld;
@@ -67,10 +69,10 @@ __global__ __launch_bounds__(256) void k_scalar_mixed(const SrcRecS32 *__restric
     }
 #pragma unroll
     for (int j = 0; j < TPL; j++) {
-        int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+        int t = tb * (TILE * TPL) + j * TILE + tid;
         if (t < n_tgt) {
-            partial[((size_t)blockIdx.y * 2) * n_tgt + t] = tr[j];
-            partial[((size_t)blockIdx.y * 2 + 1) * n_tgt + t] = ti[j];
+            partial[((size_t)sp * 2) * n_tgt + t] = tr[j];
+            partial[((size_t)sp * 2 + 1) * n_tgt + t] = ti[j];
         }
     }
 }
@@ -83,10 +85,12 @@ __global__ __launch_bounds__(256) void k_scalar_fp64(const SrcRecS64 *__restrict
                                                      double rev_scale, double *__restrict__ partial) {
     __shared__ SrcRecS64 tile[TILE];
     const int tid = threadIdx.x;
+    int tb, sp;
+    xcd_item(tb, sp);
     double px[TPL], py[TPL], pz[TPL];
 #pragma unroll
     for (int j = 0; j < TPL; j++) {
-        int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+        int t = tb * (TILE * TPL) + j * TILE + tid;
         px[j] = (t < n_tgt) ? tx[t] : 1.0e6;
         py[j] = (t < n_tgt) ? ty[t] : 1.0e6;
         pz[j] = (t < n_tgt) ? tz[t] : 1.0e6;
@@ -94,7 +98,7 @@ __global__ __launch_bounds__(256) void k_scalar_fp64(const SrcRecS64 *__restrict
     double ar[TPL], ai[TPL];
 #pragma unroll
     for (int j = 0; j < TPL; j++) { ar[j] = ai[j] = 0.0; }
-    const int s_begin = blockIdx.y * chunk;
+    const int s_begin = sp * chunk;
     const int s_end = min(n_src, s_begin + chunk);
     const uint4 *g = reinterpret_cast<const uint4 *>(src);
     TileLoader<3> ld;
@@ -126,10 +130,10 @@ __global__ __launch_bounds__(256) void k_scalar_fp64(const SrcRecS64 *__restrict
     }
 #pragma unroll
     for (int j = 0; j < TPL; j++) {
-        int t = blockIdx.x * (TILE * TPL) + j * TILE + tid;
+        int t = tb * (TILE * TPL) + j * TILE + tid;
         if (t < n_tgt) {
-            partial[((size_t)blockIdx.y * 2) * n_tgt + t] = ar[j];
-            partial[((size_t)blockIdx.y * 2 + 1) * n_tgt + t] = ai[j];
+            partial[((size_t)sp * 2) * n_tgt + t] = ar[j];
+            partial[((size_t)sp * 2 + 1) * n_tgt + t] = ai[j];
         }
     }
 }
