@@ -54,10 +54,20 @@ struct TileLoader {
 // fetches that split's records, again after each eviction.  This bijection
 // hands XCD x a contiguous run of the split-major item list instead: the
 // XCDs work on different splits and the workgroups sharing an XCD's L2 at
-// one time read the same records.  Which workgroup computes which item
-// does not change any result.
+// one time read the same records.  Only with at least kXcds splits: a
+// split's items then run on one XCD or two, and a short last split only
+// shortens one XCD's run.  With fewer, the contiguous runs would put whole
+// splits of different lengths on different XCDs (a 3-tile and a 2-tile
+// split on four XCDs each ran 1.5x slower), and the records of so few
+// tiles stay in L2 anyway.  Which workgroup computes which item does not
+// change any result.
 constexpr int kXcds = 8;
 __device__ __forceinline__ void xcd_item(int &tb, int &sp) {
+    if ((int)gridDim.y < kXcds) {
+        tb = blockIdx.x;
+        sp = blockIdx.y;
+        return;
+    }
     const int nx = gridDim.x, n = nx * gridDim.y;
     const int lin = blockIdx.y * nx + blockIdx.x;
     const int x = lin % kXcds, q = n / kXcds, r = n % kXcds;
