@@ -171,10 +171,11 @@ def _container_from_rows(rows):
     return c
 
 
-def _bundle_from_rows(rows):
-    """c2Bundle over a (12, n) float64 array in c2Bundle row order."""
+def _bundle_from_rows(rows, struct=PS.c2Bundle):
+    """Bundle (c2Bundle by default) over a float64 array whose rows follow the
+    struct's field order."""
     rows = np.ascontiguousarray(rows, dtype=np.float64)
-    b = PS.c2Bundle()
+    b = struct()
     for i, (name, _) in enumerate(b._fields_):
         setattr(b, name, rows[i].ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     b._keep = rows
@@ -183,16 +184,20 @@ def _bundle_from_rows(rows):
 
 def propagate_on_grids(lib, mode, src_rows, cells, tgt_rows, cur_rows, k, nthreads=None, epsilon=1.0,
                        exp="fwd"):
-    """JM / EH / FF from (7, n_s) source rows onto (7, n_t) target rows that
-    the caller generated (port only; no grid generation inside).  cells =
-    (gmode, ncx, ncy) of the source grid; cur_rows (12, n_s).  Returns the
-    (12, n_t) result rows in c2Bundle order."""
-    modes = {"JM": 0, "EH": 1, "FF": 4}
+    """JM / EH / JMEH / EHP / FF from (7, n_s) source rows onto (7, n_t)
+    target rows that the caller generated (port only; no grid generation
+    inside).  cells = (gmode, ncx, ncy) of the source grid; cur_rows
+    (12, n_s).  Returns the result rows in the field order of the mode's
+    bundle: (12, n_t) c2Bundle, (24, n_t) c4Bundle for JMEH, (15, n_t)
+    c2rBundle for EHP."""
+    modes = {"JM": 0, "EH": 1, "JMEH": 2, "EHP": 3, "FF": 4}
+    structs = {"JMEH": PS.c4Bundle, "EHP": PS.c2rBundle}
     nthreads = nthreads or default_threads()
     cs, ct = _container_from_rows(src_rows), _container_from_rows(tgt_rows)
     cur = _bundle_from_rows(cur_rows)
-    out = np.zeros((12, ct.size))
-    res = _bundle_from_rows(out)
+    struct = structs.get(mode, PS.c2Bundle)
+    out = np.zeros((len(struct._fields_), ct.size))
+    res = _bundle_from_rows(out, struct)
     f = lib.cdll.oracle_propagate_on_grids
     f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                   ctypes.POINTER(PS.reflcontainer), ctypes.POINTER(PS.reflcontainer),

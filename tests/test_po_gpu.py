@@ -325,6 +325,69 @@ def test_large_jm_mixed_and_chain_against_cpu(hip, port):
     assert e_chain <= 1e-5
 
 
+@pytest.mark.parametrize("mode", ["EH", "JMEH", "EHP"])
+def test_large_near_modes_against_cpu(hip, port, mode):
+    """The other near-field modes at the large configuration's full size: the
+    33 x 33 Huygens feed (J and M both non-zero) onto the whole 1001 x 1001
+    dish on the GPU (1.09e9 pairs), both precisions, checked against the FP64
+    CPU checker on every 97th dish point and the rim.  Bars: 1e-5 (mixed) and
+    1e-10 (FP64) of each field's peak; EHP's reflected Poynting directions
+    (unit vectors) to 1e-5 and 1e-10 absolute."""
+    import torch
+    import bench
+    from helpers import grid_rows, current_rows
+    from pypo_amd import device as D
+    w0 = bench.feed_waist()
+    feed, dish, _ = bench.surfaces()
+    cur_feed = bench.feed_currents(w0)
+    g_feed, g_dish = D.generate_grid(feed), D.generate_grid(dish)
+    n = bench.N_DISH ** 2
+    idx = np.unique(np.concatenate([np.arange(0, n, 97), np.arange(n - bench.N_DISH, n)]))
+    ref = OB.propagate_on_grids(port, mode, grid_rows(port, feed), (feed["gmode"], bench.N_FEED, bench.N_FEED),
+                                grid_rows(port, dish)[:, idx], current_rows(cur_feed), bench.K)
+    for prec, tol in (("mixed", 1e-5), ("fp64", 1e-10)):
+        out = D.propagate(mode, g_feed, g_dish, D.currents_tensor(cur_feed), feed, bench.K, precision=prec)
+        torch.cuda.synchronize()
+        got = out[:, idx].cpu().numpy()
+        n_field = 24 if mode == "JMEH" else 12
+        for g0 in range(0, n_field, 6):          # each complex 3-vector: rows re x y z, im x y z
+            a = got[g0:g0 + 3] + 1j * got[g0 + 3:g0 + 6]
+            r = ref[g0:g0 + 3] + 1j * ref[g0 + 3:g0 + 6]
+            peak = np.max(np.abs(r))
+            err = float(np.max(np.abs(a - r)) / peak) if peak > 0 else float(np.max(np.abs(a)))
+            print(f"large {mode} {prec} rows {g0}..{g0 + 5}: {err:.2e} of peak {peak:.3e}")
+            assert err <= tol
+        if mode == "EHP":
+            err = float(np.max(np.abs(got[12:15] - ref[12:15])))
+            print(f"large EHP {prec} Poynting directions: {err:.2e}")
+            assert err <= tol
+
+
+def test_large_scalar_against_cpu(hip, port):
+    """The scalar mode at the large configuration's size: a scalar Gaussian on
+    the 33 x 33 feed onto the whole 1001 x 1001 dish (1.09e9 pairs) in both
+    precisions against the FP64 CPU checker on every dish point."""
+    import torch
+    import bench
+    from pypo_amd import device as D
+    w0 = bench.feed_waist()
+    feed, dish, _ = bench.surfaces()
+    x = np.linspace(-bench.FEED_HALF, bench.FEED_HALF, bench.N_FEED)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    field = T.scalarfield(np.exp(-(X ** 2 + Y ** 2) / w0 ** 2) * np.exp(0.3j * X))
+    ref, _, _ = OB.propagate(port, "scalar", feed, dish, field, bench.K)
+    r = np.ravel(ref.S)
+    g_feed, g_dish = D.generate_grid(feed), D.generate_grid(dish)
+    src = D.scalar_tensor(field)
+    for prec, tol in (("mixed", 1e-5), ("fp64", 1e-10)):
+        out = D.propagate("scalar", g_feed, g_dish, src, feed, bench.K, precision=prec)
+        torch.cuda.synchronize()
+        a = out[0].cpu().numpy() + 1j * out[1].cpu().numpy()
+        err = float(np.max(np.abs(a - r)) / np.max(np.abs(r)))
+        print(f"large scalar {prec}: {err:.2e} of peak")
+        assert err <= tol
+
+
 def test_bad_arguments_raise(hip):
     src = G.plane({"gmode": "xy", "lims_x": [-1, 1], "lims_y": [-1, 1], "gridsize": [3, 3]})
     z = np.zeros((3, 3), complex)
