@@ -201,8 +201,10 @@ int pypo_hip_device_count(void);
  * are not ordered against each other).  Returns 1 when a set grows with the
  * stream-ordered allocator (old buffer freed, new one allocated in the
  * stream's order, nothing else waits), 0 when growth waits for the set's
- * stream, negative on error (no device).  Growth inside a stream capture is
- * an error: run a call once before capturing it.  No counterpart in the
+ * stream, negative on error (no device).  Device-resident calls may be
+ * captured into a graph once the same call has run on the capturing stream:
+ * inside a capture the library skips its ordering events and reports growth
+ * as an error; a replay uses that stream's buffers.  No counterpart in the
  * reference (its CPU library allocates per call). */
 int pypo_hip_stream_ordered_scratch(void);
 

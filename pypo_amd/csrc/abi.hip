@@ -60,6 +60,7 @@ hipStream_t g_cur_st = nullptr;
 PoolSet *g_open = nullptr;         // set of a call that stopped before end_call
 hipStream_t g_open_st = nullptr;
 bool g_async_alloc = false;        // stream-ordered allocation (hipMallocAsync) available
+bool g_capturing = false;          // the call being enqueued is inside a stream capture
 hipStream_t g_stream = nullptr;
 int g_fail_allocs = 0;              // fault injection: pool requests left to fail
 int g_precision = PYPO_PREC_MIXED;
@@ -96,16 +97,24 @@ int begin_call(hipStream_t st) {
     }
     PoolSet &ps = g_sets[st];
     if (!ps.done) PYPO_HIP_TRY(hipEventCreateWithFlags(&ps.done, hipEventDisableTiming));
-    else PYPO_HIP_TRY(hipStreamWaitEvent(st, ps.done, 0));
+    // Inside a stream capture the graph's own order stands in for the
+    // events: an event recorded outside the capture cannot be waited on in
+    // it, nor one recorded in it outside.  A captured call must find its
+    // buffers grown already (pool_get refuses to grow during a capture).
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    g_capturing = hipStreamIsCapturing(st, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone;
+    if (!g_capturing) PYPO_HIP_TRY(hipStreamWaitEvent(st, ps.done, 0));
     g_cur = &ps;
     g_cur_st = st;
-    g_open = &ps;
-    g_open_st = st;
+    if (!g_capturing) {
+        g_open = &ps;
+        g_open_st = st;
+    }
     return 0;
 }
 
 int end_call(hipStream_t st) {
-    PYPO_HIP_TRY(hipEventRecord(g_cur->done, st));
+    if (!g_capturing) PYPO_HIP_TRY(hipEventRecord(g_cur->done, st));
     g_open = nullptr;
     return 0;
 }
@@ -132,14 +141,13 @@ void *pool_get(int slot, size_t bytes, int *err) {
     if (ps.bytes[slot] >= bytes) return ps.buf[slot];
     const size_t want = bytes + bytes / 8;   // headroom for slightly larger calls
     hipError_t e;
+    if (g_capturing) {
+        set_error(ERR_ARG, "a scratch buffer would have to grow inside a stream capture: run the "
+                           "same call once on this stream before capturing it");
+        *err = ERR_ARG;
+        return nullptr;
+    }
     if (g_async_alloc) {
-        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-        if (hipStreamIsCapturing(g_cur_st, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone) {
-            set_error(ERR_ARG, "a scratch buffer would have to grow inside a stream capture: run the "
-                               "same call once on this stream before capturing it");
-            *err = ERR_ARG;
-            return nullptr;
-        }
         // Stream order frees the old buffer after the earlier calls on this
         // set are done with it and makes the new one ready for what follows;
         // nothing else on the device waits.

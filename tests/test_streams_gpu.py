@@ -71,3 +71,58 @@ def test_scratch_growth_in_stream_order(hip):
     assert torch.equal(a, ref_s)
     assert torch.equal(b, ref_l)
     assert torch.equal(c, ref_s)
+
+
+@pytest.mark.parametrize("no_alloc", [False, True])
+@pytest.mark.parametrize("mode", ["JM", "FF"])
+def test_propagation_captured_in_a_graph(hip, mode, no_alloc):
+    """A device-resident propagation captured into a graph after one eager
+    call on the capturing stream: every replay, with unrelated eager work and
+    synchronisation between replays, equals the eager result."""
+    import torch
+    from pypo_amd import device as D
+    src, tgt, cur = _surfaces(37, 41)
+    if mode == "FF":
+        tgt = G.plane({"gmode": "AoE", "lims_Az": [-3, 3], "lims_El": [-3, 3], "gridsize": [43, 47]})
+    k = 2 * np.pi / 2.0
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        gs, gt = D.generate_grid(src), D.generate_grid(tgt)
+        c = D.currents_tensor(cur)
+        ref = D.propagate(mode, gs, gt, c, src, k, no_alloc=no_alloc).clone()   # warms the stream's buffers
+        out = torch.empty_like(ref)
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        D.propagate(mode, gs, gt, c, src, k, out=out, no_alloc=no_alloc)
+    for rep in range(3):
+        out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), rep
+        junk = torch.randn(4096, device="cuda").cumsum(0)     # unrelated eager work between replays
+        torch.cuda.synchronize()
+        assert torch.isfinite(junk).all()
+
+
+def test_capture_that_would_grow_scratch_is_refused(hip):
+    """Growing a buffer set inside a capture is reported, not attempted."""
+    import torch
+    from pypo_amd import bind as B
+    from pypo_amd import device as D
+    src, tgt, cur = _surfaces(61, 67)
+    k = 2 * np.pi / 2.0
+    gs, gt = D.generate_grid(src), D.generate_grid(tgt)
+    c = D.currents_tensor(cur)
+    out = torch.empty((12, gt.shape[1]), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()          # fresh stream: its buffer set is empty
+    g = torch.cuda.CUDAGraph()
+    with pytest.raises(B.PyPOHIPError, match="inside a stream capture"):
+        with torch.cuda.graph(g, stream=s):
+            D.propagate("EH", gs, gt, c, src, k, out=out)
+    torch.cuda.synchronize()
+    again = D.propagate("EH", gs, gt, c, src, k)
+    torch.cuda.synchronize()
+    assert torch.isfinite(again).all()
