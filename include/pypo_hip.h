@@ -245,11 +245,12 @@ int pypo_hip_propagate_dev(int mode, void *res, const reflcontainer *cs,
  * A problem whose packed sources plus one split of partials exceed the
  * scratch is rejected with ERR_ARG (1001) before anything runs.  Every call
  * on a device shares its scratch, which keeps the last call's contents: the
- * library orders its calls with one completion event, so calls from several
- * threads or streams run one after another; work replayed outside the
- * library's ordering (e.g. from a captured graph) would race on it.  Calls
- * no hipMalloc/hipFree; the first library call of a process creates the
- * library's stream and completion event. */
+ * library orders these calls with one completion event per device, so calls
+ * from several threads or streams run one after another; replays of a
+ * captured call are outside that ordering and race with any other no-alloc
+ * work on the device that overlaps them.  Calls no hipMalloc/hipFree; the
+ * first call on a device creates that device's completion event (and the
+ * first library call of a process the library's stream). */
 int pypo_hip_propagate_noalloc(int mode, void *res, const reflcontainer *cs,
                                const reflcontainer *ct, const void *currents,
                                int gmode, int ncx, int ncy, double k, double epsilon,

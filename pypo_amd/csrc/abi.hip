@@ -853,11 +853,12 @@ int pypo_hip_propagate_dev(int mode, void *res, const reflcontainer *cs, const r
 
 namespace {
 constexpr int kMaxDevices = 64;
-void *g_scratch[kMaxDevices] = {};   // device address of each device's scratch
+void *g_scratch[kMaxDevices] = {};        // device address of each device's scratch
+hipEvent_t g_scratch_done[kMaxDevices] = {};   // completion of the last no-alloc call on each device
 
-// This device's scratch address, fetched on the first call on the device
-// with that device current.
-int scratch_of_current_device(void **out) {
+// This device's scratch address and ordering event, made on the first call
+// on the device with that device current.
+int scratch_of_current_device(void **out, hipEvent_t **done) {
     int dev = -1;
     PYPO_HIP_TRY(hipGetDevice(&dev));
     PYPO_REQUIRE(dev >= 0 && dev < kMaxDevices, ERR_ARG, "no-alloc propagation: device index out of range");
@@ -867,9 +868,22 @@ int scratch_of_current_device(void **out) {
         PYPO_REQUIRE(p != nullptr, ERR_ARG, "no-alloc propagation: scratch symbol has no device address");
         g_scratch[dev] = p;
     }
+    if (!g_scratch_done[dev]) PYPO_HIP_TRY(hipEventCreateWithFlags(&g_scratch_done[dev], hipEventDisableTiming));
     *out = g_scratch[dev];
+    *done = &g_scratch_done[dev];
     return 0;
 }
+
+// Records the scratch's completion event on the call's stream when the call
+// ends, also when it stops on an error after enqueuing work.
+struct ScratchMark {
+    hipEvent_t ev;
+    hipStream_t st;
+    bool on;
+    ~ScratchMark() {
+        if (on) (void)hipEventRecord(ev, st);
+    }
+};
 }  // namespace
 
 int pypo_hip_propagate_noalloc(int mode, void *res, const reflcontainer *cs, const reflcontainer *ct,
@@ -899,7 +913,8 @@ int pypo_hip_propagate_noalloc(int mode, void *res, const reflcontainer *cs, con
         return ERR_ARG;
     }
     void *base = nullptr;
-    if ((err = scratch_of_current_device(&base))) return err;
+    hipEvent_t *scratch_done = nullptr;
+    if ((err = scratch_of_current_device(&base, &scratch_done))) return err;
     const long long fit = (long long)((cap - pk) / per_split);
     const SumPlan plan = plan_cap_splits(plan_sum(mode, precision, gs, gt), gs, (int)std::min(fit, 1LL << 30));
     double *partial = (double *)((char *)base + pk);
@@ -907,6 +922,10 @@ int pypo_hip_propagate_noalloc(int mode, void *res, const reflcontainer *cs, con
     PYPO_REQUIRE(partial_bytes(mode, plan, gt) <= wsb, ERR_ARG, "internal: split plan exceeds the scratch");
     hipStream_t st = (hipStream_t)stream;   // NULL: the default stream
     if ((err = begin_call(st))) return err;
+    // Calls on every stream of the device share its scratch: each waits for
+    // the previous one (outside a capture, whose replays are not ordered).
+    if (!g_capturing) PYPO_HIP_TRY(hipStreamWaitEvent(st, *scratch_done, 0));
+    ScratchMark mark{*scratch_done, st, !g_capturing};
     GridPtrs gsrc = grid_ptrs(*cs);
     if (mode == PYPO_MODE_SCALAR) {
         const arrC1 *f = (const arrC1 *)currents;

@@ -97,3 +97,28 @@ def test_noalloc_large_far_field_and_rejection(hip):
     assert torch.all(sentinel == 3.0)
     inp = _inputs("EH", 21, 17, 9)
     assert torch.equal(_run("EH", "mixed", inp, True), _run("EH", "mixed", inp, False))
+
+
+def test_noalloc_calls_on_two_streams_share_the_scratch_in_turn(hip):
+    """No-alloc calls on two streams, enqueued with no host synchronisation,
+    share the device's one scratch: the library orders them, so each equals
+    its pypo_hip_propagate_dev result (overlapping calls would overwrite each
+    other's packed sources and partials)."""
+    import torch
+    from pypo_amd import device as D
+    cases = [("FF", 57, 61, 11), ("EH", 43, 37, 12), ("FF", 31, 45, 13), ("JM", 49, 29, 14)]
+    refs, ins = [], []
+    for mode, ns, nt, seed in cases:
+        src, gs, gt, vals = _inputs(mode, ns, nt, seed)
+        ins.append((mode, src, gs, gt, vals))
+        refs.append(D.propagate(mode, gs, gt, vals, src, 2 * np.pi / 3.0).clone())
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    outs = []
+    for rep in range(2):
+        for i, (mode, src, gs, gt, vals) in enumerate(ins):
+            with torch.cuda.stream(s1 if (i + rep) % 2 == 0 else s2):
+                outs.append((i, D.propagate(mode, gs, gt, vals, src, 2 * np.pi / 3.0, no_alloc=True)))
+    torch.cuda.synchronize()
+    for i, o in outs:
+        assert torch.equal(o, refs[i]), cases[i]
