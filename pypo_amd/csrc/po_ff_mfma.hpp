@@ -87,7 +87,9 @@ __device__ __forceinline__ float tile_inv_scale(float cmax) {
 // Issue cost per wave-instruction measured on gfx950 (4 waves per SIMD):
 // v_cvt_pk_f16_f32 5.4 cycles, v_fma_mix_f32 5.5, but v_fma_mix{lo,hi}_f16
 // 9.9 -- so this four-instruction form (21.8) beats cvt_pk + mixlo + mixhi
-// (25.2), with bit-identical lo (checked on 2^24 inputs).
+// (25.2), with bit-identical lo (checked on 2^24 inputs).  In the large far
+// field the same holds: the mixlo + mixhi form gave bit-identical sums in
+// 99.5 ms against 91.0.
 __device__ __forceinline__ void split_pair_fast(float c, float s, unsigned &hi, unsigned &lo) {
     hi = __builtin_bit_cast(unsigned, half2v{(_Float16)c, (_Float16)s});
     float dc, ds;
