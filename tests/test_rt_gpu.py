@@ -116,3 +116,45 @@ def test_empty_frame_is_a_no_op(hip):
     one = T.frame(1, np.zeros(1), np.zeros(1), np.ones(1), np.zeros(1), np.zeros(1), -np.ones(1))
     hit = B.RT_HIPd({"fr_in": one, "t_name": pla, "tol": 1e-12, "t0": 1.0})
     np.testing.assert_allclose([hit.z[0], hit.dz[0]], [0.0, 1.0], atol=1e-12)
+
+
+def test_ellipsoid_and_plane_full_size_against_cpu(hip, port):
+    """1e7 rays (3162 x 3162 directions over a 0.8 rad cone) from the first
+    focus of an ellipsoid (foci (0, 0, 1000) and (0, 0, 0) mm, ecc 0.5)
+    reflected by it and then by a plane at z = 200 mm tilted 10 degrees,
+    both surfaces in one device launch, FP64, tol 1e-12, against the CPU
+    checker surface by surface; every ray leaving the ellipsoid passes
+    through its second focus."""
+    import torch
+    from pypo_amd import device as D
+    n1 = 3162
+    g = np.linspace(-0.4, 0.4, n1)
+    AX, AY = np.meshgrid(g, g, indexing="ij")
+    d = np.stack([np.sin(AX.ravel()), np.sin(AY.ravel()), np.ones(AX.size)])
+    d /= np.linalg.norm(d, axis=0)
+    z = np.zeros(AX.size)
+    fr = T.frame(AX.size, z, z, np.full(AX.size, 1000.0), d[0], d[1], d[2])
+    ell = G.ellipsoid({"pmode": "focus", "focus_1": [0, 0, 1000.0], "focus_2": [0, 0, 0.0], "ecc": 0.5,
+                       "gmode": "xy", "lims_x": [-800, 800], "lims_y": [-800, 800], "gridsize": [3, 3]})
+    tilt = np.radians(10.0)
+    pla = G.plane({"gmode": "xy", "lims_x": [-500, 500], "lims_y": [-500, 500], "gridsize": [3, 3]})
+    pla["transf"] = G.mat_translate([0, 0, 200.0], G.find_rotation(np.array([0, 0, 1.0]),
+                                                                   np.array([0, np.sin(tilt), np.cos(tilt)])))
+    t_fr = D.to_device(np.stack([getattr(fr, a) for a in COMP]))
+    f1, f2 = D.trace([ell, pla], t_fr, tol=1e-12, t0=500.0)
+    torch.cuda.synchronize()
+    g1, g2 = f1.cpu().numpy(), f2.cpu().numpy()
+    r1 = OB.trace(port, ell, fr, None, 1e-12, 500.0, max_iter=50)
+    r2 = OB.trace(port, pla, r1, None, 1e-12, 500.0, max_iter=50)
+    assert not np.isnan(g2).any()
+    e1 = np.max(np.abs(g1[:3] - _stack(r1)[:3]))
+    e2 = np.max(np.abs(g2[:3] - _stack(r2)[:3]))
+    d1 = np.max(np.abs(g1[3:] - _stack(r1)[3:]))
+    d2 = np.max(np.abs(g2[3:] - _stack(r2)[3:]))
+    print(f"ellipsoid + plane hit-point max diff: {e1:.2e}, {e2:.2e} mm; directions {d1:.2e}, {d2:.2e}")
+    assert e1 <= 1e-9 and e2 <= 1e-9 and d1 <= 1e-12 and d2 <= 1e-12
+    # closed form: the reflected rays pass through the second focus (0, 0, 0)
+    s = -g1[2] / g1[5]
+    miss = np.hypot(g1[0] + s * g1[3], g1[1] + s * g1[4])
+    print(f"ellipsoid: max distance of the reflected rays from the second focus {np.max(miss):.2e} mm")
+    assert np.max(miss) < 1e-6
