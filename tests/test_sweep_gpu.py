@@ -64,6 +64,31 @@ def test_launch_shapes_agree(small, tpl, chunk, wgs):
     s.plan(tpl=2, chunk_tiles=2, wgs_per_cu=2, use_graph=False)
 
 
+@pytest.mark.parametrize("tpl", [1, 2])
+def test_vector_kernel_sweep_matches_checker(small, port, tpl):
+    """The far-field stage on the vector ALUs (k_sweep_ff, selected with
+    pypo_hip_set_ff_kernel) against the CPU checker and the matrix-core stage."""
+    from pypo_amd import bind as B
+    s, (feeds, dishes, ks, curs, ff, meta) = small
+    s.plan(tpl=2, chunk_tiles=2, wgs_per_cu=2, use_graph=False)
+    s.run()
+    matrix = [flat(s.read(i)) for i in range(s.n_inst)]
+    B.set_ff_kernel("vector")
+    try:
+        s.plan(tpl=tpl, chunk_tiles=2, wgs_per_cu=3, use_graph=False)
+        s.run()
+        for i in range(s.n_inst):
+            jm_ref, _, _ = OB.propagate(port, "JM", feeds[i], dishes[i], curs[i], ks[i])
+            ff_ref, _, _ = OB.propagate(port, "FF", dishes[i], ff, jm_ref, ks[i])
+            got = flat(s.read(i))
+            e_ref, e_mat = group_err(got, flat(ff_ref)), group_err(got, matrix[i])
+            print(f"vector sweep tpl {tpl} instance {i}: vs checker {e_ref:.2e}, vs matrix {e_mat:.2e}")
+            assert e_ref <= 1e-5 and e_mat <= 1e-5
+    finally:
+        B.set_ff_kernel("matrix")
+        s.plan(tpl=2, chunk_tiles=2, wgs_per_cu=2, use_graph=False)
+
+
 def test_graph_replays_match_eager(small):
     import torch
     s, _ = small
