@@ -670,7 +670,17 @@ class System:
                 res.setMeta(name_target, field.k)
                 self.fields[name_out] = res
                 return res
-            comps = [c.reshape(shape) for c in D.tensor_to_complex6(out)]
+            # the other five components are zero (System._compToFields); the
+            # result stays on the device and only the one component is copied
+            import torch
+            rows = [6 * (ci // 3) + ci % 3, 6 * (ci // 3) + 3 + ci % 3]
+            kept = torch.zeros_like(out)
+            kept[rows] = out[rows]
+            res = T.fields.on_device(kept, shape)
+            res.setMeta(name_target, field.k)
+            self.fields[name_out] = res
+            a = out[rows].cpu().numpy()
+            return (a[0] + 1j * a[1]).reshape(shape)
         else:
             from scipy.interpolate import griddata
             g = grid.cpu().numpy()
