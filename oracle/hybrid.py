@@ -13,9 +13,10 @@ from scipy.interpolate import griddata
 from . import bind as OB
 
 
-def carry(lib, target, fr_in, comps, k, start=None, tol=1e-3, t0=1.0):
-    """Trace fr_in onto target; return (fr_out, carried components (6, n))."""
-    fr_out = OB.trace(lib, target, fr_in, tol=tol, t0=t0)
+def carry(lib, target, fr_in, comps, k, start=None, tol=1e-3, t0=1.0, max_iter=0):
+    """Trace fr_in onto target; return (fr_out, carried components (6, n)).
+    max_iter caps the Newton iterations (0: no cap, as the reference)."""
+    fr_out = OB.trace(lib, target, fr_in, tol=tol, t0=t0, max_iter=max_iter)
     L1 = np.sqrt((fr_out.x - fr_in.x) ** 2 + (fr_out.y - fr_in.y) ** 2 + (fr_out.z - fr_in.z) ** 2)
     if start is not None:
         L0 = np.sqrt((fr_in.x - start[0]) ** 2 + (fr_in.y - start[1]) ** 2 + (fr_in.z - start[2]) ** 2)
