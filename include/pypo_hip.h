@@ -208,6 +208,21 @@ int pypo_hip_device_count(void);
  * reference (its CPU library allocates per call). */
 int pypo_hip_stream_ordered_scratch(void);
 
+/* Device bytes of scratch the library holds for calls on `stream` (a
+ * hipStream_t, NULL: the null stream); 0 for a stream it has not seen,
+ * negative on error (no device).  No counterpart in the reference. */
+long long pypo_hip_scratch_bytes(void *stream);
+
+/* Frees the scratch buffers the library holds for `stream`, for callers
+ * that use many short-lived streams (the sets are grow-only and otherwise
+ * kept until the process ends).  Waits until the last call enqueued on the
+ * stream outside a capture is done with them (the stream is not passed to
+ * HIP, so it may already be destroyed).  A graph captured from calls on the
+ * stream uses these buffers: do not call this during such a capture or while
+ * such a graph may still be launched.  The next call on the stream
+ * allocates a new set.  Returns 0 on success. */
+int pypo_hip_release_scratch(void *stream);
+
 /* Precision used by the drop-in PO entry points (default PYPO_PREC_MIXED).
  * Returns the previous value. */
 int pypo_hip_set_precision(int precision);
@@ -223,9 +238,9 @@ int pypo_hip_set_rt_max_iter(int max_iter);
  * c2rBundle (EHP) or arrC1 (scalar) according to mode.  gmode/ncx/ncy are
  * the source grid's gmode and n_cells (trapezoid weights,
  * Propagation.h:1123-1135).  stream is a hipStream_t; NULL means the
- * default (null) stream.  Asynchronous on `stream`; calls are ordered after
- * every earlier library call whatever their streams (they share the
- * library's workspace).  Returns 0 on success. */
+ * default (null) stream.  Asynchronous on `stream`, in that stream's order;
+ * calls on different streams use different scratch buffers and are not
+ * ordered against each other.  Returns 0 on success. */
 int pypo_hip_propagate_dev(int mode, void *res, const reflcontainer *cs,
                            const reflcontainer *ct, const void *currents,
                            int gmode, int ncx, int ncy, double k, double epsilon,

@@ -80,6 +80,10 @@ def loadHIPlib():
         lib.pypo_hip_device_count.restype = i
         lib.pypo_hip_stream_ordered_scratch.argtypes = []
         lib.pypo_hip_stream_ordered_scratch.restype = i
+        lib.pypo_hip_scratch_bytes.argtypes = [ctypes.c_void_p]
+        lib.pypo_hip_scratch_bytes.restype = ctypes.c_longlong
+        lib.pypo_hip_release_scratch.argtypes = [ctypes.c_void_p]
+        lib.pypo_hip_release_scratch.restype = i
         lib.pypo_hip_set_precision.argtypes = [i]
         lib.pypo_hip_set_precision.restype = i
         lib.pypo_hip_set_rt_max_iter.argtypes = [i]

@@ -487,6 +487,44 @@ int pypo_hip_stream_ordered_scratch(void) {
     return g_async_alloc ? 1 : 0;
 }
 
+long long pypo_hip_scratch_bytes(void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return -err;
+    auto it = g_sets.find((hipStream_t)stream);
+    if (it == g_sets.end()) return 0;
+    long long total = 0;
+    for (int s = 0; s < SLOT_COUNT; s++) total += (long long)it->second.bytes[s];
+    return total;
+}
+
+int pypo_hip_release_scratch(void *stream) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int err = ensure_init();
+    if (err) return err;
+    auto it = g_sets.find((hipStream_t)stream);
+    if (it == g_sets.end()) return 0;
+    PoolSet &ps = it->second;
+    if (g_open == &ps) {   // a call on it stopped before end_call: mark its end first
+        (void)hipEventRecord(ps.done, g_open_st);
+        g_open = nullptr;
+    }
+    // Only the library's own event and stream are used: the caller's stream
+    // may be gone.  After the event, no call outside a capture uses the set.
+    PYPO_HIP_TRY(hipEventSynchronize(ps.done));
+    for (int s = 0; s < SLOT_COUNT; s++) {
+        if (!ps.buf[s]) continue;
+        PYPO_HIP_TRY(g_async_alloc ? hipFreeAsync(ps.buf[s], g_stream) : hipFree(ps.buf[s]));
+        ps.buf[s] = nullptr;
+        ps.bytes[s] = 0;
+    }
+    if (g_async_alloc) PYPO_HIP_TRY(hipStreamSynchronize(g_stream));
+    PYPO_HIP_TRY(hipEventDestroy(ps.done));
+    if (g_cur == &ps) g_cur = nullptr;
+    g_sets.erase(it);
+    return 0;
+}
+
 int pypo_hip_set_precision(int precision) {
     std::lock_guard<std::recursive_mutex> lock(g_mu);
     int old = g_precision;

@@ -271,6 +271,28 @@ class GridCache:
         self._grids.clear()
 
 
+def scratch_bytes(stream=None):
+    """Device bytes of scratch the library holds for calls on `stream` (a
+    torch.cuda.Stream; None: the current stream)."""
+    lib = B.loadHIPlib()
+    st = ctypes.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
+    n = lib.pypo_hip_scratch_bytes(st)
+    if n < 0:
+        B.check(lib, "pypo_hip_scratch_bytes")
+    return int(n)
+
+
+def release_scratch(stream=None):
+    """Free the scratch the library holds for calls on `stream` (None: the
+    current stream), e.g. before dropping a short-lived stream.  Waits for the
+    stream's last library call; graphs captured from calls on the stream must
+    not be launched afterwards."""
+    lib = B.loadHIPlib()
+    st = ctypes.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
+    if lib.pypo_hip_release_scratch(st):
+        B.check(lib, "pypo_hip_release_scratch")
+
+
 def tensor_to_frame(t):
     from . import types as T
     a = t.detach().cpu().numpy()

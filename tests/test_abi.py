@@ -162,4 +162,6 @@ def test_new_entry_points_report_errors_without_a_device():
         BG.PyPO_GPUf(src, src, {"mode": "FF", "k": 1.0, "s_current": T.currents(z, z, z, z, z, z)},
                      interface="cpu")
     assert lib.pypo_hip_stream_ordered_scratch() < 0
+    assert lib.pypo_hip_scratch_bytes(None) < 0
+    assert lib.pypo_hip_release_scratch(None) != 0
     lib.pypo_hip_last_error(buf, 256)

@@ -4,6 +4,8 @@ not ordered against each other, and a set that must grow frees and allocates
 in its stream's order (no device-wide synchronisation).  Results must be
 bit-identical to the same calls made one after another on one stream."""
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -71,6 +73,38 @@ def test_scratch_growth_in_stream_order(hip):
     assert torch.equal(a, ref_s)
     assert torch.equal(b, ref_l)
     assert torch.equal(c, ref_s)
+
+
+def test_release_scratch_of_a_stream(hip):
+    """A stream's scratch is freed on request, also after the stream itself
+    was dropped, and the next call on a stream allocates a new set with the
+    same results; other streams keep theirs."""
+    import torch
+    from pypo_amd import device as D
+    case = _surfaces(41, 37)
+    ref = _run(D, case, "EH", "mixed").clone()
+    torch.cuda.synchronize()
+    keep = D.scratch_bytes()
+    assert keep > 0
+    s = torch.cuda.Stream()
+    assert D.scratch_bytes(s) == 0
+    with torch.cuda.stream(s):
+        a = _run(D, case, "EH", "mixed")
+    held = D.scratch_bytes(s)
+    assert held > 0
+    D.release_scratch(s)          # waits for the call queued on s
+    assert D.scratch_bytes(s) == 0
+    assert torch.equal(a, ref)
+    with torch.cuda.stream(s):
+        b = _run(D, case, "EH", "mixed")
+    s.synchronize()
+    assert torch.equal(b, ref) and D.scratch_bytes(s) == held
+    handle = s.cuda_stream
+    del s
+    assert hip.pypo_hip_release_scratch(ctypes.c_void_p(handle)) == 0   # its stream may be gone
+    assert hip.pypo_hip_scratch_bytes(ctypes.c_void_p(handle)) == 0
+    assert hip.pypo_hip_release_scratch(ctypes.c_void_p(handle)) == 0   # nothing held: no-op
+    assert D.scratch_bytes() == keep
 
 
 @pytest.mark.parametrize("no_alloc", [False, True])
