@@ -71,3 +71,74 @@ def test_injected_failure_in_device_and_ray_paths(hip):
     assert lib.pypo_hip_inject_alloc_failures(-1) != 0
     buf = ctypes.create_string_buffer(128)
     assert lib.pypo_hip_last_error(buf, 128) != 0 and b"inject" in buf.value   # read and cleared
+
+
+def test_device_entry_points_reject_bad_arguments(hip):
+    """Argument errors of the device-resident entry points come back as a
+    non-zero code and a message, before anything is enqueued, and the
+    library stays usable: the next good call matches a clean run."""
+    import ctypes
+    import torch
+    from pypo_amd import device as D
+    from pypo_amd import efficiencies as E
+    from pypo_amd import marshal as MU
+    from pypo_amd import structs as PS
+    lib = hip
+    src, tgt, d = _case()
+    k = d["k"]
+    gs, gt = D.generate_grid(src), D.generate_grid(tgt)
+    cur = D.currents_tensor(d["s_current"])
+    ref = D.propagate("EH", gs, gt, cur, src, k).clone()
+    out = torch.zeros_like(ref)
+    cs, ct, cb, rb = D.container(gs), D.container(gt), D.bundle(cur), D.bundle(out)
+    st = D._stream()
+    buf = ctypes.create_string_buffer(512)
+
+    def vp(x):
+        return ctypes.cast(ctypes.byref(x), ctypes.c_void_p)
+
+    def expect(rc, text):
+        assert rc != 0
+        assert lib.pypo_hip_last_error(buf, 512) != 0
+        assert text in buf.value.decode(), buf.value
+
+    lib.pypo_hip_last_error(buf, 512)
+    args = [vp(rb), ctypes.byref(cs), ctypes.byref(ct), vp(cb), 0, 21, 23, k, 1.0, -1.0]
+    for fn in (lib.pypo_hip_propagate_dev, lib.pypo_hip_propagate_noalloc):
+        expect(fn(99, *args, 0, st), "unknown PO mode")
+        expect(fn(1, *args, 5, st), "unknown precision")
+        expect(fn(1, *args[:5], 20, 23, *args[7:], 0, st), "ncx*ncy")
+        expect(fn(1, None, *args[1:], 0, st), "NULL argument")
+    # more packed sources than the no-alloc scratch holds: refused before any launch
+    n_big = 2000 * 1000
+    g_big = torch.zeros((7, n_big), dtype=torch.float64, device="cuda")
+    c_big = torch.zeros((12, n_big), dtype=torch.float64, device="cuda")
+    cs_big, cb_big = D.container(g_big), D.bundle(c_big)
+    expect(lib.pypo_hip_propagate_noalloc(1, vp(rb), ctypes.byref(cs_big), ctypes.byref(ct), vp(cb_big), 0, 2000,
+                                          1000, k, 1.0, -1.0, 0, st), "more than its")
+    del g_big, c_big, cs_big, cb_big
+    # ray tracing takes 1 to 4 surfaces
+    surf = MU.fill_reflparams(src)
+    surfs = (PS.reflparams * 5)(*([surf] * 5))
+    fr = torch.zeros((6, 8), dtype=torch.float64, device="cuda")
+    frames = (PS.cframe * 6)(*([D.frame_struct(fr)] * 6))
+    for n_surf in (0, 5):
+        expect(lib.pypo_hip_trace_dev(surfs, n_surf, frames, 1e-9, 1.0, st), "1 to 4 surfaces")
+    # nearest interpolation of an empty point set; reductions over nothing
+    D._declare_hybrid(lib)
+    vals = torch.zeros((12, 8), dtype=torch.float64, device="cuda")
+    res = torch.zeros((12, gt.shape[1]), dtype=torch.float64, device="cuda")
+    pe, ve, re_ = D.frame_struct(fr), D.bundle(vals), D.bundle(res)
+    pe.size = 0     # valid arrays, no points
+    expect(lib.pypo_hip_interp_nearest_dev(ctypes.byref(pe), ctypes.byref(ve), ctypes.byref(ct), ctypes.byref(re_),
+                                           st), "empty point set")
+    E._declare(lib)
+    sums = (ctypes.c_double * 8)()
+    row = ctypes.cast(ctypes.c_void_p(ref[0].data_ptr()), ctypes.POINTER(ctypes.c_double))
+    expect(lib.pypo_hip_power_sums_dev(row, row, row, row, 0, sums, st), "n must be positive")
+    expect(lib.pypo_hip_frame_stats_dev(ctypes.byref(pe), sums, st), "empty frame")
+    # nothing was enqueued and the library still works
+    D.propagate("EH", gs, gt, cur, src, k, out=out)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    assert lib.pypo_hip_last_error(buf, 512) == 0
