@@ -251,6 +251,75 @@ def run_sweep(torch, log):
                         "201^2..401^2; feed 25x25; AoE 181^2 +-1deg"}
 
 
+def hybrid_system():
+    """The hybrid section of PyPO's Cassegrain tutorial (tutorials/Tutorial5.ipynb
+    cell 19) as a System, lambda = 2 mm: a Gaussian (w0 = 12 mm) on a 101 x 100
+    polar plane at z = -1000 mm; the hyperboloid secondary (foci (0, 0, 3500)
+    and (0, 0, -1000) mm, ecc 1.1, u up to 300 mm) on a 2501 x 2500 uv grid;
+    the paraboloid primary (focus (0, 0, 3500) mm); a 1501 x 1500 polar plane
+    at z = 3500 mm over the primary's aperture (u 300..5000 mm)."""
+    from pypo_amd import geometry as G
+    from pypo_amd.system import System
+    s = System()
+    s.addParabola({"name": "primary", "pmode": "focus", "gmode": "uv", "vertex": np.zeros(3),
+                   "focus_1": np.array([0, 0, 3.5e3]), "lims_u": np.array([300, 5e3]),
+                   "lims_v": np.array([0, 360]), "gridsize": np.array([320, 40])})
+    s.addHyperbola({"name": "secondary", "pmode": "focus", "gmode": "uv", "flip": True,
+                    "focus_1": np.array([0, 0, 3.5e3]), "focus_2": np.array([0, 0, -1e3]), "ecc": 1.1,
+                    "lims_u": np.array([0, 300]), "lims_v": np.array([0, 360]), "gridsize": np.array([2501, 2500])})
+    s.addPlane({"name": "focus", "gmode": "uv", "lims_u": np.array([0, 100]), "lims_v": np.array([0, 360]),
+                "gridsize": np.array([101, 100]), "transf": G.mat_translate([0, 0, -1e3])})
+    s.addPlane({"name": "foc_pri", "gmode": "uv", "lims_u": np.array([300, 5e3]), "lims_v": np.array([0, 360]),
+                "gridsize": np.array([1501, 1500]), "transf": G.mat_translate([0, 0, 3.5e3])})
+    s.createGaussian({"name": "source", "lam": 2.0, "w0x": 12.0, "w0y": 12.0, "n": 1, "E0": 1, "dxyz": 0,
+                      "pol": np.array([1, 0, 0])}, "focus")
+    return s
+
+
+HYBRID_START = np.array([0, 0, 3.5e3])   # rays leave the secondary as from the primary focus
+
+
+def hybrid_calls(s):
+    """The three System calls of the hybrid section: EHP onto the secondary,
+    the reflected rays to the primary (start-point amplitude, no
+    interpolation), then to the focal plane with nearest interpolation of Ex."""
+    def ehp():
+        s.runPO({"t_name": "secondary", "s_current": "source", "mode": "EHP", "name_EH": "EH_sec",
+                 "name_P": "P_sec"})
+
+    def to_primary():
+        s.runHybridPropagation({"fr_in": "P_sec", "field_in": "EH_sec", "t_name": "primary", "fr_out": "fr_pri",
+                                "field_out": "f_pri", "start": HYBRID_START, "interp": False})
+
+    def to_plane():
+        s.runHybridPropagation({"fr_in": "fr_pri", "field_in": "f_pri", "t_name": "foc_pri", "fr_out": "fr_foc",
+                                "field_out": "f_foc", "comp": "Ex"})
+    return ehp, to_primary, to_plane
+
+
+def run_hybrid(torch, log):
+    """The hybrid section timed call by call through System (median of 3
+    synchronised calls after a warm pass, results left in device memory)."""
+    s = hybrid_system()
+    calls = hybrid_calls(s)
+    for c in calls:
+        c()
+    torch.cuda.synchronize()
+    (t_e, r_e), (t_1, r_1), (t_2, r_2) = [median_time(c, 3) for c in calls]
+    n_rays = int(np.prod(s.system["secondary"]["gridsize"]))
+    n_grid = int(np.prod(s.system["foc_pri"]["gridsize"]))
+    n_src = int(np.prod(s.system["focus"]["gridsize"]))
+    log(f"hybrid: EHP {t_e * 1e3:.1f} ms, rays to the primary {t_1 * 1e3:.1f} ms, "
+        f"rays to the plane + nearest {t_2 * 1e3:.1f} ms")
+    return {"ms_ehp": round(t_e * 1e3, 2), "ms_ehp_range": r_e, "ehp_pairs": n_src * n_rays,
+            "ms_hybrid_trace": round(t_1 * 1e3, 2), "ms_hybrid_trace_range": r_1,
+            "ms_hybrid_trace_nearest": round(t_2 * 1e3, 2), "ms_hybrid_trace_nearest_range": r_2,
+            "rays": n_rays, "grid_points": n_grid,
+            "workload": "Cassegrain hybrid section: Gaussian 101x100 -> EHP -> hyperboloid uv 2501x2500, "
+                        "rays -> paraboloid (start-point amplitude), rays -> plane 1501x1500 + nearest Ex; "
+                        "lambda=2mm; System calls, results on the device"}
+
+
 def run_adjoint(torch, log):
     """Differentiated far field: 601^2 dish (D = 3 m, f = 1.2 m, lambda = 1
     mm) with a 36-mode Zernike displacement of RMS 20 um, 201^2 far field over
@@ -397,8 +466,11 @@ def main():
     ap.add_argument("--no-rt", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-hybrid", action="store_true")
     ap.add_argument("--profile-adjoint", action="store_true",
                     help="only the adjoint forward + backward (for rocprofv3 kernel statistics)")
+    ap.add_argument("--profile-hybrid", action="store_true",
+                    help="only the hybrid section's three System calls, warm then once more (for rocprofv3)")
     ap.add_argument("--profile-sweep", action="store_true",
                     help="only the batched sweep at its tuned launch shape (for rocprofv3)")
     ap.add_argument("--profile", action="store_true",
@@ -421,6 +493,14 @@ def main():
 
     if args.profile_adjoint:
         run_adjoint(torch, log)
+        return
+    if args.profile_hybrid:
+        calls = hybrid_calls(hybrid_system())
+        for _ in range(2):
+            for c in calls:
+                c()
+        torch.cuda.synchronize()
+        log("hybrid profile run done")
         return
     if args.profile_sweep:
         from pypo_amd import sweep as SW
@@ -563,6 +643,8 @@ def main():
         res["sweep"] = run_sweep(torch, log)
     if not args.no_adjoint:
         res["adjoint"] = run_adjoint(torch, log)
+    if not args.no_hybrid:
+        res["hybrid"] = run_hybrid(torch, log)
 
     line = {
         "metric": "PO pair-interactions/s JM->FF (mixed FP32/FP64 vs FP64) at <=1e-5 peak err; rays/s",

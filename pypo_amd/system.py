@@ -655,6 +655,12 @@ class System:
         on the host with the frame's points and the grid generated on the
         device (griddata supports "linear" on these 3-D points and raises for
         "cubic", as it does for the reference)."""
+        return self._interp_frame(name_fr_in, name_field, name_target, name_out, comp, method, True)
+
+    def _interp_frame(self, name_fr_in, name_field, name_target, name_out, comp, method, host_copy):
+        """interpFrame; with host_copy False the one-component form skips
+        copying that component to the host for the return value (the hybrid
+        step discards it) and returns None."""
         from . import device as D
         if name_fr_in not in self.frames:
             raise RunRTError(f"Frame {name_fr_in} not in system.")
@@ -679,6 +685,8 @@ class System:
             res = T.fields.on_device(kept, shape)
             res.setMeta(name_target, field.k)
             self.fields[name_out] = res
+            if not host_copy:
+                return None
             a = out[rows].cpu().numpy()
             return (a[0] + 1j * a[1]).reshape(shape)
         else:
@@ -755,7 +763,7 @@ class System:
         carried.setMeta(d["t_name"], field.k)
         self.fields[d["field_out"]] = carried
         if d["interp"]:
-            self.interpFrame(d["fr_out"], d["field_out"], d["t_name"], d["field_out"], comp=d["comp"])
+            self._interp_frame(d["fr_out"], d["field_out"], d["t_name"], d["field_out"], d["comp"], "nearest", False)
         self.assoc[d["t_name"]] = [d["field_out"], d["fr_out"]]
         return self.fields[d["field_out"]]
 

@@ -146,22 +146,9 @@ def test_tutorial_cassegrain_hybrid_full_size(hip, port):
     and every interpolated Ex either the value of the checker's nearest point
     or, where two points lie at the same distance to 1e-9 mm, of one of them."""
     from scipy.spatial import cKDTree
-    from pypo_amd.system import System
-    lam = 2.0
-    k = 2 * np.pi / lam
-    s = System()
-    s.addParabola({"name": "primary", "pmode": "focus", "gmode": "uv", "vertex": np.zeros(3),
-                   "focus_1": np.array([0, 0, 3.5e3]), "lims_u": np.array([300, 5e3]),
-                   "lims_v": np.array([0, 360]), "gridsize": np.array([320, 40])})
-    s.addHyperbola({"name": "secondary", "pmode": "focus", "gmode": "uv", "flip": True,
-                    "focus_1": np.array([0, 0, 3.5e3]), "focus_2": np.array([0, 0, -1e3]), "ecc": 1.1,
-                    "lims_u": np.array([0, 300]), "lims_v": np.array([0, 360]), "gridsize": np.array([2501, 2500])})
-    s.addPlane({"name": "focus", "gmode": "uv", "lims_u": np.array([0, 100]), "lims_v": np.array([0, 360]),
-                "gridsize": np.array([101, 100]), "transf": G.mat_translate([0, 0, -1e3])})
-    s.addPlane({"name": "foc_pri", "gmode": "uv", "lims_u": np.array([300, 5e3]), "lims_v": np.array([0, 360]),
-                "gridsize": np.array([1501, 1500]), "transf": G.mat_translate([0, 0, 3.5e3])})
-    s.createGaussian({"name": "source", "lam": lam, "w0x": 12.0, "w0y": 12.0, "n": 1, "E0": 1, "dxyz": 0,
-                      "pol": np.array([1, 0, 0])}, "focus")
+    import bench
+    k = 2 * np.pi / 2.0
+    s = bench.hybrid_system()
     s.runPO({"t_name": "secondary", "s_current": "source", "mode": "EHP", "name_EH": "EH_sec", "name_P": "P_sec"})
     start = np.array([0, 0, 3.5e3])
     s.runHybridPropagation({"fr_in": "P_sec", "field_in": "EH_sec", "t_name": "primary", "fr_out": "fr_pri",
