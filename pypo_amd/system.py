@@ -18,8 +18,11 @@ createUniformSource and their scalar twins 2146-2311), calcRayLen
 nearest neighbour on the GPU, scipy's other methods on the host); the frame and beam figures of merit
 (calcRTcenter / calcRTtilt / calcSpotRMS 1694-1746, calcSpillover / calcTaper
 / calcXpol 1748-1826) as GPU reductions; findRTfocus (2632-2700) over GPU
-traces; and the grid-convergence loops autoConverge / convergeOnTarget
-(2779-3080) over GPU scalar propagations.
+traces; the grid-convergence loops autoConverge / convergeOnTarget
+(2779-3080) over GPU scalar propagations; and the bookkeeping around them:
+mergeBeams (1195-1241, summed on the device), interpBeam (2306-2353, scipy
+on the host as there), removeFrame / removeField / removeCurrent (994-1040),
+setOverride, setLoggingVerbosity and copyObj.
 """
 
 import copy
@@ -46,6 +49,22 @@ class RunRTError(Exception):
 
 class HybridPropError(Exception):
     """A hybridDict is invalid (Checks.py:1604-1654)."""
+
+
+class FieldNameError(Exception):
+    """No PO field of that name in the system (Checks.py:85-110)."""
+
+
+class CurrentNameError(Exception):
+    """No PO current of that name in the system (Checks.py:112-137)."""
+
+
+class FrameNameError(Exception):
+    """No ray-trace frame of that name in the system (Checks.py:166-191)."""
+
+
+class MergeBeamError(Exception):
+    """Beams to merge are defined on different surfaces (Checks.py:1755-1777)."""
 
 
 FIELD_COMPONENTS = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
@@ -777,6 +796,99 @@ class System:
 
     def removeScalarField(self, name):
         self.scalarfields.pop(name, None)
+
+    def removeFrame(self, frameName):
+        """System.removeFrame (System.py:994-1008)."""
+        if frameName not in self.frames:
+            raise FrameNameError(f"Frame {frameName} not in system.")
+        del self.frames[frameName]
+
+    def removeField(self, fieldName):
+        """System.removeField (System.py:1010-1024)."""
+        if fieldName not in self.fields:
+            raise FieldNameError(f"Field {fieldName} not in system.")
+        del self.fields[fieldName]
+
+    def removeCurrent(self, currentName):
+        """System.removeCurrent (System.py:1026-1040)."""
+        if currentName not in self.currents:
+            raise CurrentNameError(f"Current {currentName} not in system.")
+        del self.currents[currentName]
+
+    def setOverride(self, override=True):
+        """Overwrite results of an existing name (True) or append _n (False)
+        (System.setOverride, System.py:231-242)."""
+        self.override = bool(override)
+
+    def setLoggingVerbosity(self, verbose=True):
+        """Log at DEBUG (True) or only CRITICAL (False) (System.py:213-229)."""
+        self.verbosity = verbose
+        log.setLevel(logging.DEBUG if verbose else logging.CRITICAL)
+
+    def copyObj(self, obj=None):
+        """Deep copy of obj, or of the system (System.copyObj, System.py:2702-2712);
+        a result held in device memory is copied in device memory."""
+        return copy.deepcopy(self if obj is None else obj)
+
+    def _beam_pool(self, obj):
+        """"fields" / "currents" pool for an Enums.Objects value or its string."""
+        key = getattr(obj, "value", obj)
+        if key not in ("fields", "currents"):
+            raise ValueError(f"obj must be fields or currents, not {key}")
+        return key, getattr(self, key)
+
+    def mergeBeams(self, *beams, obj="fields", merged_name="combined"):
+        """Sum of PO fields or currents defined on one surface, stored under
+        merged_name (System.mergeBeams, System.py:1195-1241; obj is
+        Enums.Objects.FIELD / CURRENT or "fields" / "currents").  The sum runs
+        on the device and the result stays there."""
+        key, pool = self._beam_pool(obj)
+        missing = [b for b in beams if b not in pool]
+        if missing:
+            err = FieldNameError if key == "fields" else CurrentNameError
+            raise err(f"{missing[0]} not in system {key}.")
+        surf0 = pool[beams[0]].surf
+        other = [b for b in beams if pool[b].surf != surf0]
+        if other:
+            raise MergeBeamError(f"{other[0]} is defined on {pool[other[0]].surf}, not on {surf0}.")
+        from . import device as D
+        ex = pool[beams[0]]
+        total = D.currents_tensor(ex).clone()
+        for b in beams[1:]:
+            total += D.currents_tensor(pool[b])
+        cls = T.fields if key == "fields" else T.currents
+        res = cls.on_device(total, ex.shape)
+        res.setMeta(ex.surf, ex.k)
+        pool[merged_name] = res
+
+    def interpBeam(self, name, gridsize_new, obj="fields"):
+        """Linear interpolation of a PO field or current on a planar surface onto
+        a copy of that surface with gridsize_new, stored as name + "_interp" on
+        the surface surf + "_interp" (System.interpBeam, System.py:2306-2353).
+        As in the reference, scipy's griddata interpolates real and imaginary
+        parts over the grids' (x, y); the grids are generated on the GPU."""
+        from scipy.interpolate import griddata
+        key, pool = self._beam_pool(obj)
+        if name not in pool:
+            err = FieldNameError if key == "fields" else CurrentNameError
+            raise err(f"{name} not in system {key}.")
+        src = pool[name]
+        surf_new = src.surf + "_interp"
+        self.copyElement(src.surf, surf_new)
+        self.system[surf_new]["gridsize"] = np.asarray(gridsize_new)
+        g0, g1 = self.generateGrids(src.surf), self.generateGrids(surf_new)
+        points = (np.ravel(g0.x), np.ravel(g0.y))
+        points_new = (np.ravel(g1.x), np.ravel(g1.y))
+        shape = tuple(int(v) for v in np.asarray(gridsize_new))
+        comps = []
+        for c in src.components():
+            c = np.ravel(c)
+            re = griddata(points, np.real(c), points_new)
+            im = griddata(points, np.imag(c), points_new)
+            comps.append((re + 1j * im).reshape(shape))
+        res = (T.fields if key == "fields" else T.currents)(*comps)
+        res.setMeta(surf_new, src.k)
+        pool[name + "_interp"] = res
 
     def _strongest_component(self, source_field):
         """The source's component with the largest magnitude, as a scalarfield

@@ -230,3 +230,75 @@ def test_runPO_chains_on_device(hip, monkeypatch):
     s.runPO({"t_name": "ff", "mode": "FF", "s_current": "dJM", "name_EH": "ffEH2"})
     ff2 = B.PyPO_HIPd(s.system["dish"], s.system["ff"], {"mode": "FF", "k": k, "s_current": jm_host})
     np.testing.assert_array_equal(flat(s.fields["ffEH2"]), flat(ff2))
+
+
+@pytest.mark.gpu
+def test_merge_interp_and_remove_beams(hip):
+    """mergeBeams, interpBeam and the remove* / setOverride / copyObj helpers
+    (the reference's test_mergeBeams and test_interpBeam,
+    tests/test_SystemPO_RT.py:198-214, plus values): a merge is the sum of
+    its beams and stays on the device; beams on different surfaces are
+    refused; linear interpolation of a field linear in (x, y) returns that
+    field on the new grid."""
+    from pypo_amd.system import CurrentNameError, FieldNameError, FrameNameError, MergeBeamError
+    s = make_system()
+    g = {"name": "gauss", "lam": 0.8, "w0x": 1.0, "w0y": 1.2, "n": 1.0, "E0": 1.0, "dxyz": 0.0,
+         "pol": np.array([0.0, 1.0, 0.0])}
+    s.createGaussian(g, "feed")
+    s.createUniformSource({"name": "us", "lam": 0.8, "E0": 1.0, "phase": 0.3, "pol": np.array([1.0, 0.0, 0.0])},
+                          "feed")
+    for obj in ("fields", "currents"):
+        s.mergeBeams("gauss", "us", obj=obj, merged_name="test")
+        pool = getattr(s, obj)
+        assert pool["test"].device_rows() is not None and pool["test"].surf == "feed"
+        want = flat(pool["gauss"]) + flat(pool["us"])
+        np.testing.assert_allclose(flat(pool["test"]), want, rtol=0, atol=1e-15 * np.max(np.abs(want)))
+    s.createUniformSource({"name": "far", "lam": 0.8, "E0": 1.0, "phase": 0.0, "pol": np.array([1.0, 0.0, 0.0])},
+                          "dish")
+    with pytest.raises(MergeBeamError):
+        s.mergeBeams("gauss", "far")
+    with pytest.raises(FieldNameError):
+        s.mergeBeams("gauss", "nope")
+    # a field linear in (x, y) on the 17 x 17 feed plane, interpolated onto 101 x 101
+    grid = s.generateGrids("feed")
+    x, y = np.asarray(grid.x), np.asarray(grid.y)
+    lin = [(a + 0.5 * x - 2.0 * y) + 1j * (b - x + 0.25 * y) for a, b in
+           ((1, 0), (0, 1), (2, 2), (-1, 0), (0, -3), (0.5, 0.5))]
+    s.fields["lin"] = T.fields(*lin)
+    s.fields["lin"].setMeta("feed", 2 * np.pi / 0.8)
+    s.interpBeam("lin", np.array([101, 101]))
+    assert s.system["feed_interp"]["gridsize"].tolist() == [101, 101]
+    assert s.fields["lin_interp"].surf == "feed_interp"
+    g2 = s.generateGrids("feed_interp")
+    x2, y2 = np.asarray(g2.x), np.asarray(g2.y)
+    inner = (slice(1, -1), slice(1, -1))
+    for i, (a, b) in enumerate(((1, 0), (0, 1), (2, 2), (-1, 0), (0, -3), (0.5, 0.5))):
+        want = (a + 0.5 * x2 - 2.0 * y2) + 1j * (b - x2 + 0.25 * y2)
+        got = s.fields["lin_interp"].components()[i]
+        assert np.all(np.isfinite(got[inner]))
+        ok = np.isfinite(got)
+        np.testing.assert_allclose(got[ok], want[ok], rtol=0, atol=1e-12)
+    s.interpBeam("gauss", np.array([31, 29]), obj="currents")
+    assert s.currents["gauss_interp"].shape == (31, 29)
+    # remove*, with the reference's name errors
+    s.removeField("lin")
+    s.removeCurrent("gauss_interp")
+    with pytest.raises(FieldNameError):
+        s.removeField("lin")
+    with pytest.raises(CurrentNameError):
+        s.removeCurrent("gauss_interp")
+    with pytest.raises(FrameNameError):
+        s.removeFrame("nothing")
+    s.createTubeFrame({"name": "tube", "nRays": 4, "nRing": 2, "angx0": 1.0, "angy0": 1.0, "x0": 2.0, "y0": 2.0})
+    s.removeFrame("tube")
+    assert "tube" not in s.frames
+    # copyObj copies a device-resident result; setOverride(False) suffixes names
+    c = s.copyObj(s.fields["test"])
+    np.testing.assert_array_equal(flat(c), flat(s.fields["test"]))
+    s.setOverride(False)
+    s.mergeBeams("gauss", "us", merged_name="test")     # merge stores under the name given, as there
+    s.createUniformSource({"name": "us", "lam": 0.8, "E0": 1.0, "phase": 0.0, "pol": np.array([1.0, 0.0, 0.0])},
+                          "feed")
+    assert "us_1" in s.fields
+    s.setLoggingVerbosity(False)
+    s.setLoggingVerbosity(True)
