@@ -478,3 +478,37 @@ def test_near_modes_random_cases(hip, port, seed):
         err = group_err(flat(out), flat(ref))
         print(f"seed {seed} {mode} {prec}: src {nx}x{ny}, lam {lam:.3g}: {err:.2e}")
         assert err <= TOL[prec]
+
+
+@pytest.mark.parametrize("shape", [(2, 2), (1, 5), (5, 1), (1, 1)])
+def test_degenerate_source_grids_follow_the_checker(hip, port, shape):
+    """Source grids with one or two points per axis.  A one-point xy axis
+    divides its extent by zero in the reference's generator (NaN coordinates,
+    infinite areas; a uv grid with one v point stays finite); the GPU grids,
+    and every mode's result, have the same non-finite entries as the CPU
+    checker and agree with it elsewhere."""
+    rng = np.random.default_rng(sum(shape))
+    tgt = G.plane({"gmode": "xy", "lims_x": [-3, 3], "lims_y": [-3, 3], "gridsize": [4, 3]})
+    tgt["transf"] = G.mat_translate([0, 0, 10.0])
+    ff = G.plane({"gmode": "AoE", "lims_Az": [-5, 5], "lims_El": [-5, 5], "gridsize": [5, 4]})
+    sources = [G.plane({"gmode": "xy", "lims_x": [-2, 2], "lims_y": [-2, 2], "gridsize": list(shape)}),
+               G.paraboloid({"pmode": "focus", "vertex": [0, 0, 0], "focus_1": [0, 0, 60.0], "gmode": "uv",
+                             "lims_u": [0, 25], "lims_v": [0, 360], "gridsize": list(shape)})]
+    for src in sources:
+        g_ref = OB.generate_grid(port, src)
+        g_got = B.generate_grid(src)
+        for a in ("x", "y", "z", "area"):
+            r, o = np.ravel(getattr(g_ref, a)), np.ravel(getattr(g_got, a))
+            np.testing.assert_array_equal(np.isfinite(o), np.isfinite(r))
+            np.testing.assert_allclose(o[np.isfinite(r)], r[np.isfinite(r)], rtol=1e-13, atol=1e-12)
+        cur = T.currents(*[rng.normal(size=shape) + 1j * rng.normal(size=shape) for _ in range(6)])
+        for mode, target in (("EH", tgt), ("JM", tgt), ("FF", ff)):
+            ref, _, _ = OB.propagate(port, mode, src, target, cur, 2 * np.pi / 1.5)
+            r = flat(ref)
+            for prec in ("mixed", "fp64"):
+                o = flat(B.PyPO_HIPd(src, target, {"mode": mode, "k": 2 * np.pi / 1.5, "s_current": cur,
+                                                   "precision": prec}))
+                fin = np.isfinite(r)
+                np.testing.assert_array_equal(np.isfinite(o), fin, err_msg=f"{src['gmode']} {mode} {prec}")
+                if fin.all():
+                    assert group_err(o, r) <= TOL[prec], (src["gmode"], mode, prec)
