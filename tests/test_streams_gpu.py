@@ -50,6 +50,77 @@ def test_calls_on_two_streams_match_one_stream(hip, prec):
         assert torch.equal(o, ref[i]), (i, modes[i])
 
 
+def test_calls_from_several_host_threads(hip):
+    """Four host threads, each on its own stream, mix device-resident calls,
+    synchronous drop-in calls and a call that fails on a bad argument.  The
+    library serialises the host side and keeps scratch per stream and the
+    error state per thread: every result is bit-identical to the same call
+    made alone, and each thread sees only its own error."""
+    import threading
+    import torch
+    from helpers import flat
+    from pypo_amd import bind as B
+    from pypo_amd import device as D
+    cases = [_surfaces(31, 41), _surfaces(57, 23), _surfaces(19, 77), _surfaces(45, 45)]
+    modes = ["EH", "JM", "FF", "JMEH"]
+    k = 2 * np.pi / 2.0
+    ref_dev = [_run(D, c, m, "mixed").clone() for c, m in zip(cases, modes)]
+    ref_drop = [flat(B.PyPO_HIPd(c[0], c[1], {"mode": "EH", "k": k, "s_current": c[2]})) for c in cases]
+    torch.cuda.synchronize()
+    errors, outs = [], {}
+
+    def worker(i):
+        try:
+            st = torch.cuda.Stream()
+            with torch.cuda.stream(st):
+                for rep in range(3):
+                    j = (i + rep) % len(cases)
+                    dev = _run(D, cases[j], modes[j], "mixed")
+                    drop = flat(B.PyPO_HIPd(cases[j][0], cases[j][1], {"mode": "EH", "k": k,
+                                                                       "s_current": cases[j][2]}))
+                    if rep == 1:
+                        with pytest.raises(B.PyPOHIPError, match="unknown PO mode"):
+                            _bad_mode(D, cases[j], k)
+                    st.synchronize()
+                    outs[(i, rep)] = (j, dev.clone(), drop)
+        except BaseException as e:   # reported on the main thread
+            errors.append((i, repr(e)))
+
+    threads = [threading.Thread(target=worker, args=(i,), daemon=True) for i in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(120)
+    assert not errors, errors
+    assert len(outs) == 12
+    for (i, rep), (j, dev, drop) in outs.items():
+        assert torch.equal(dev, ref_dev[j]), (i, rep, modes[j])
+        np.testing.assert_array_equal(drop, ref_drop[j])
+
+
+def _bad_mode(D, case, k):
+    """A device call with an unknown mode number, through the C ABI."""
+    import ctypes
+    from pypo_amd import bind as B
+    lib = B.loadHIPlib()
+    src, tgt, cur = case
+    gs, gt = D.generate_grid(src), D.generate_grid(tgt)
+    c = D.currents_tensor(cur)
+    out = torch_empty_like_rows(gt, 12)
+    cs, ct, cb, rb = D.container(gs), D.container(gt), D.bundle(c), D.bundle(out)
+    ncx, ncy = (int(v) for v in src["gridsize"])
+    rc = lib.pypo_hip_propagate_dev(77, ctypes.cast(ctypes.byref(rb), ctypes.c_void_p), ctypes.byref(cs),
+                                    ctypes.byref(ct), ctypes.cast(ctypes.byref(cb), ctypes.c_void_p), 0, ncx, ncy,
+                                    k, 1.0, -1.0, 0, D._stream())
+    assert rc != 0
+    B.check(lib, "pypo_hip_propagate_dev")
+
+
+def torch_empty_like_rows(t, rows):
+    import torch
+    return torch.empty((rows, t.shape[1]), dtype=torch.float64, device=t.device)
+
+
 def test_scratch_grows_with_the_stream_ordered_allocator(hip):
     assert hip.pypo_hip_stream_ordered_scratch() == 1
 
