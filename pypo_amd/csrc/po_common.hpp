@@ -114,9 +114,20 @@ struct Cone {
     float ax, ay, az, cos_half;
 };
 
-// The cone holding the unit vectors (vx, vy, vz) of the lanes with
-// valid == true, over the workgroup.
+// The cone holding the directions of the vectors (vx, vy, vz) of the lanes
+// with valid == true, over the workgroup.  Each vector is normalised first:
+// grid normals are unit only under an orthonormal transform (a transf whose
+// rotation part scales gives normals of other lengths), and a cone built
+// from longer vectors would be too narrow for the n.r sign tests it bounds.
+// Zero and non-finite vectors are left as they are (they widen the cone to
+// the whole sphere, so every per-pair test stays on).
 __device__ __forceinline__ Cone block_cone(float vx, float vy, float vz, bool valid, float *scratch) {
+    const float big = fmaxf(fabsf(vx), fmaxf(fabsf(vy), fabsf(vz)));
+    if (valid && big > 0.f && big < INFINITY) {
+        vx /= big; vy /= big; vz /= big;   // largest component 1: no overflow or underflow below
+        const float s = 1.f / sqrtf(vx * vx + vy * vy + vz * vz);
+        vx *= s; vy *= s; vz *= s;
+    }
     float sx = block_sum(valid ? vx : 0.f, scratch);
     float sy = block_sum(valid ? vy : 0.f, scratch);
     float sz = block_sum(valid ? vz : 0.f, scratch);

@@ -413,7 +413,7 @@ def test_kernel_timing_api(hip):
         lib.pypo_hip_set_timing(0)
 
 
-@pytest.mark.parametrize("seed", range(24))
+@pytest.mark.parametrize("seed", list(range(24)) + [44])   # 44: normals of length 1.025 (shadow cones)
 def test_ff_matrix_random_cases(hip, port, seed):
     """Seeded random far-field cases for the matrix and the vector kernel
     against the CPU checker: xy or uv source grids of random shape (partial last tiles, row
@@ -512,3 +512,38 @@ def test_degenerate_source_grids_follow_the_checker(hip, port, shape):
                 np.testing.assert_array_equal(np.isfinite(o), fin, err_msg=f"{src['gmode']} {mode} {prec}")
                 if fin.all():
                     assert group_err(o, r) <= TOL[prec], (src["gmode"], mode, prec)
+
+
+def test_ff_shadow_test_with_scaled_normals(hip, port):
+    """A transf whose rotation part scales (1.5 R) gives grid normals of
+    length 1.5; the reference's shadow test n.r < 0 only reads their sign.
+    The kernels skip that test per tile only where the tile's normal cone and
+    the targets' direction cone prove it rejects nothing, and those cones are
+    built from directions, so every kernel still agrees with the checker.
+    Built from the length-1.5 normals, the tile cones came out too narrow
+    and every kernel, FP64 included, skipped tests that reject pairs here
+    (0.22-0.28 of peak off).  0.4 % of the pairs are shadowed over the
+    +-40 deg far field, and no pair has |n.r| below 9e-6, so the FP32
+    per-pair test decides as the reference's FP64 one does."""
+    src = G.paraboloid({"pmode": "focus", "vertex": [0, 0, 0], "focus_1": [0, 0, 30.0], "gmode": "uv",
+                        "lims_u": [0, 40], "lims_v": [0, 360], "gridsize": [40, 36]})
+    a = np.radians(21.0)
+    rot = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+    m = np.eye(4)
+    m[:3, :3] = 1.5 * rot
+    m[:3, 3] = [2.0, -1.0, 3.0]
+    src["transf"] = m
+    ff = G.plane({"gmode": "AoE", "lims_Az": [-40, 40], "lims_El": [-40, 40], "gridsize": [61, 61]})
+    rng = np.random.default_rng(7)
+    cur = T.currents(*[rng.normal(size=(40, 36)) + 1j * rng.normal(size=(40, 36)) for _ in range(6)])
+    k = 2 * np.pi / 2.0
+    ref, _, _ = OB.propagate(port, "FF", src, ff, cur, k)
+    try:
+        for kind, prec in (("matrix", "mixed"), ("vector", "mixed"), ("matrix", "fp64")):
+            B.set_ff_kernel(kind)
+            out = B.PyPO_HIPd(src, ff, {"mode": "FF", "k": k, "s_current": cur, "precision": prec})
+            err = group_err(flat(out), flat(ref))
+            print(f"scaled normals, {kind} {prec}: {err:.2e}")
+            assert err <= TOL[prec], (kind, prec, err)
+    finally:
+        B.set_ff_kernel("matrix")
