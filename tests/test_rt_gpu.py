@@ -158,3 +158,90 @@ def test_ellipsoid_and_plane_full_size_against_cpu(hip, port):
     miss = np.hypot(g1[0] + s * g1[3], g1[1] + s * g1[4])
     print(f"ellipsoid: max distance of the reflected rays from the second focus {np.max(miss):.2e} mm")
     assert np.max(miss) < 1e-6
+
+
+def _random_surface(rng):
+    """A random quadric or plane (xy or uv grid, maybe flipped) under a random
+    rigid transform, sometimes with a scaling rotation part."""
+    t = int(rng.integers(4))
+    n = [3, 3]
+    flip = bool(rng.random() < 0.3)
+    xy = rng.random() < 0.5
+    if t == 0:
+        a, b = rng.uniform(20, 80, 2)
+        s = G.paraboloid({"gmode": "xy" if xy else "uv", "lims_x": [-15, 15], "lims_y": [-12, 12],
+                          "lims_u": [0, 15], "lims_v": [0, 360], "pmode": "manual", "coeffs": [a, b, -1],
+                          "gridsize": n, "flip": flip})
+    elif t == 1:
+        s = G.hyperboloid({"gmode": "xy" if xy else "uv", "lims_x": [-10, 10], "lims_y": [-10, 10],
+                           "lims_u": [0, 10], "lims_v": [0, 360], "pmode": "manual",
+                           "coeffs": list(rng.uniform(10, 40, 3)), "gridsize": n, "flip": flip})
+    elif t == 2:
+        a, b, c = rng.uniform(20, 40, 3)
+        s = G.ellipsoid({"gmode": "uv", "lims_u": [0, 0.7 * min(a, b)], "lims_v": [0, 360], "pmode": "manual",
+                         "coeffs": [a, b, c], "gridsize": n, "flip": flip})
+    else:
+        s = G.plane({"gmode": "xy" if xy else "uv", "lims_x": [-10, 10], "lims_y": [-8, 8], "lims_u": [0, 10],
+                     "lims_v": [0, 360], "gridsize": n, "flip": flip})
+    ax = rng.normal(size=3)
+    m = G.mat_translate(rng.uniform(-20, 20, 3), G.find_rotation(np.array([0, 0, 1.0]), ax / np.linalg.norm(ax)))
+    if rng.random() < 0.3:
+        m[:3, :3] *= rng.uniform(0.6, 1.8)
+    s["transf"] = m @ s["transf"]
+    return s
+
+
+def _surface_residual(surf, p):
+    """|f| of the surface equation at points p (3, n), in the surface's own frame."""
+    minv = np.linalg.inv(surf["transf"])
+    x, y, z = minv[:3, :3] @ p + minv[:3, 3:4]
+    a, b, c = surf["coeffs"]
+    f = {0: lambda: z - (x ** 2 / a ** 2 + y ** 2 / b ** 2),
+         1: lambda: z - c * np.sqrt(x ** 2 / a ** 2 + y ** 2 / b ** 2 + 1),
+         2: lambda: z - c * np.sqrt(np.maximum(0.0, 1 - x ** 2 / a ** 2 - y ** 2 / b ** 2)),
+         3: lambda: z}[int(surf["type"])]()
+    return np.abs(f)
+
+
+@pytest.mark.parametrize("seed", range(80))
+def test_rt_random_surfaces_and_frames(hip, port, seed):
+    """Random surfaces of every type under random (sometimes scaling)
+    transforms, hit by up to 3000 rays from a sphere of radius 150 mm aimed
+    near the surface, random tol and t0, against the CPU checker (both capped
+    at 50 Newton steps).  The NaN pattern is the same.  Every ray that the
+    checker lands on the surface (|f| < 1e-6), the GPU lands on it too, and
+    where that hit is well conditioned (|t| < 1e4 mm, |d.n| > 1e-3 against
+    the unit normal) at the same point to 1e-9 mm + tol and the same
+    direction to 1e-12 + tol.  (Rays the checker leaves off the surface -- misses and Newton
+    runs that do not converge in 50 steps -- end at their last iterate, which
+    rounding differences move arbitrarily; grazing and very distant hits
+    agree only to the conditioning of their root.)"""
+    rng = np.random.default_rng(9000 + seed)
+    surf = _random_surface(rng)
+    nr = int(rng.integers(1, 3000))
+    o = surf["transf"][:3, 3]
+    start = rng.normal(size=(3, nr))
+    start = o[:, None] + 150 * start / np.linalg.norm(start, axis=0)
+    d = o[:, None] + rng.normal(scale=8.0, size=(3, nr)) - start
+    d /= np.linalg.norm(d, axis=0)
+    fr = T.frame(nr, start[0], start[1], start[2], d[0], d[1], d[2])
+    tol, t0 = float(10 ** rng.uniform(-12, -6)), float(rng.uniform(1, 200))
+    ref = _stack(OB.trace(port, surf, fr, None, tol, t0, max_iter=50))
+    got = _stack(B.RT_HIPd({"fr_in": fr, "t_name": surf, "tol": tol, "t0": t0}))
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+    fin = np.isfinite(ref).all(0) & np.isfinite(got).all(0)
+    on_ref = np.zeros(nr, bool)
+    on_ref[fin] = _surface_residual(surf, ref[:3, fin]) < 1e-6
+    on_got = np.zeros(nr, bool)
+    on_got[fin] = _surface_residual(surf, got[:3, fin]) < 1e-6
+    assert np.all(on_got[on_ref]), np.flatnonzero(on_ref & ~on_got)[:5]
+    # conditioning of each hit: distance along the ray and incidence against the reflected direction
+    t = np.einsum("in,in->n", ref[:3] - start, d)
+    cos_inc = np.sqrt(np.abs(np.einsum("in,in->n", ref[3:] - d, d)) / 2)   # |d.n|: d' = d - 2 (d.n) n
+    good = on_ref & (np.abs(t) < 1e4) & (cos_inc > 1e-3)
+    print(f"seed {seed}: type {surf['type']}, {nr} rays, {np.sum(on_ref)} on the surface, {np.sum(good)} well "
+          f"conditioned, tol {tol:.1e}")
+    if good.any():
+        # the Newton loop stops once a step is <= tol, so two correct runs may stop a step apart
+        assert np.max(np.abs(got[:3, good] - ref[:3, good])) <= 1e-9 + tol
+        assert np.max(np.abs(got[3:, good] - ref[3:, good])) <= 1e-12 + tol
