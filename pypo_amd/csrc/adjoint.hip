@@ -332,8 +332,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         if (MB > 1 && cone.cos_half > 0.f) {
             float d = 1.f;
 #pragma unroll
-            for (int m = 1; m < MB; m++)
-                if (s0 + 16 * m + row < n_src) d = fminf(d, fx[m] * cone.ax + fy[m] * cone.ay + fz[m] * cone.az);
+            for (int m = 1; m < MB; m++) {
+                float ux = fx[m], uy = fy[m], uz = fz[m];   // directions: the normals need not be unit
+                to_unit(ux, uy, uz);
+                if (s0 + 16 * m + row < n_src) d = fminf(d, ux * cone.ax + uy * cone.ay + uz * cone.az);
+            }
             cone.cos_half = fminf(cone.cos_half, block_min(d, scratch));
         }
         pbx = -block_min(-mx, scratch); pby = -block_min(-my, scratch); pbz = -block_min(-mz, scratch);

@@ -114,20 +114,26 @@ struct Cone {
     float ax, ay, az, cos_half;
 };
 
-// The cone holding the directions of the vectors (vx, vy, vz) of the lanes
-// with valid == true, over the workgroup.  Each vector is normalised first:
+// Scales (vx, vy, vz) to unit length; a zero or non-finite vector is left
+// as it is.  Cones bound n.r sign tests, so they are built from directions:
 // grid normals are unit only under an orthonormal transform (a transf whose
-// rotation part scales gives normals of other lengths), and a cone built
-// from longer vectors would be too narrow for the n.r sign tests it bounds.
-// Zero and non-finite vectors are left as they are (they widen the cone to
-// the whole sphere, so every per-pair test stays on).
-__device__ __forceinline__ Cone block_cone(float vx, float vy, float vz, bool valid, float *scratch) {
+// rotation part scales gives normals of other lengths), and a cone built or
+// widened with longer vectors would come out too narrow.
+__device__ __forceinline__ void to_unit(float &vx, float &vy, float &vz) {
     const float big = fmaxf(fabsf(vx), fmaxf(fabsf(vy), fabsf(vz)));
-    if (valid && big > 0.f && big < INFINITY) {
+    if (big > 0.f && big < INFINITY) {
         vx /= big; vy /= big; vz /= big;   // largest component 1: no overflow or underflow below
         const float s = 1.f / sqrtf(vx * vx + vy * vy + vz * vz);
         vx *= s; vy *= s; vz *= s;
     }
+}
+
+// The cone holding the directions of the vectors (vx, vy, vz) of the lanes
+// with valid == true, over the workgroup (each vector made unit first; zero
+// and non-finite ones widen the cone to the whole sphere, so every per-pair
+// test stays on).
+__device__ __forceinline__ Cone block_cone(float vx, float vy, float vz, bool valid, float *scratch) {
+    if (valid) to_unit(vx, vy, vz);
     float sx = block_sum(valid ? vx : 0.f, scratch);
     float sy = block_sum(valid ? vy : 0.f, scratch);
     float sz = block_sum(valid ? vz : 0.f, scratch);

@@ -240,3 +240,38 @@ def test_full_size_gradient_against_finite_differences(full):
         worst = np.abs(g[idx] - fd).max() / np.abs(fd).max()
         print(f"{name} adjoint vs central differences on 100 points: {rel:.2e} (norm), {worst:.2e} (max/peak)")
         assert rel <= 1e-4 and worst <= 1e-4
+
+
+def test_gradient_with_scaled_dish_normals(hip, port):
+    """A dish whose transf scales by 1.23 (grid normals of length 1.23) and a
+    +-53.5 deg far field that puts 1.3 % of the (dish point, target) pairs in
+    the shadow test (the smallest |n.r| is 1.4e-4, far from rounding): the
+    matrix-core reverse sum skips a tile's per-pair tests only where the
+    dish-normal and target-direction cones exclude shadowing, and those
+    cones are built and widened from directions, so the mixed gradient of
+    both reverse sums and the FP64 one agree with the checker.  Built from
+    the longer normals, the cones came out too narrow: 335 of 625 dish
+    points off by up to 4e-2 of the largest gradient."""
+    from pypo_amd import bind as B
+    from helpers import grid_rows
+    c = small_shape_case(port, seed=29, sigma=0.02933749125690907, ff_deg=53.50493480875001)
+    m = np.eye(4)
+    m[:3, :3] *= 1.228784237485987
+    c["dish"]["transf"] = m @ c["dish"]["transf"]
+    c["dr"] = grid_rows(port, c["dish"])
+    prob = PA.ShapeProblem(c["feed"], c["cur"], c["dish"], c["ff"], c["k"])
+    d = torch.tensor(c["d"], dtype=torch.float64, device="cuda")
+    E, _ = OA.forward(port, c["fr"], c["fcells"], c["fc"], c["dr"], c["dcells"], c["d"], c["ffr"], c["k"])
+    g = np.random.default_rng(29).standard_normal(E.shape[1]) / np.abs(E).max() ** 2
+    ref = OA.power_grad(c["fr"], c["fcells"], c["fc"], OA.displace(c["dr"], c["d"]), c["dcells"], c["ffr"],
+                        E, g, c["k"])
+    try:
+        for kind, prec, bar in (("matrix", "mixed", 1e-5), ("vector", "mixed", 1e-5), ("matrix", "fp64", 1e-10)):
+            B.set_ff_kernel(kind)
+            got = prob.power_grad(d, torch.tensor(E, device="cuda"), torch.tensor(g, device="cuda"),
+                                  precision=prec).cpu().numpy()
+            worst = np.abs(got - ref).max() / np.abs(ref).max()
+            print(f"scaled dish, {kind} {prec}: largest error {worst:.2e} of the largest gradient")
+            assert worst <= bar, (kind, prec, worst)
+    finally:
+        B.set_ff_kernel("matrix")
