@@ -26,6 +26,11 @@ struct GridParams {
 
 __global__ __launch_bounds__(256) void k_generate_grid(GridParams P, GridPtrs g, int n,
                                                        const double *__restrict__ errs) {
+    // No fused multiply-adds: each product and sum rounds on its own, as in
+    // the reference's host build, so xy grids come out bit-identical to it
+    // (a contracted j * dy + ylo leaves 1.8e-15 where the reference has an
+    // exact 0, and atan2 at the origin then takes another branch).
+#pragma clang fp contract(off)
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
     int i = idx / P.ny, j = idx - i * P.ny;

@@ -245,3 +245,36 @@ def test_rt_random_surfaces_and_frames(hip, port, seed):
         # the Newton loop stops once a step is <= tol, so two correct runs may stop a step apart
         assert np.max(np.abs(got[:3, good] - ref[:3, good])) <= 1e-9 + tol
         assert np.max(np.abs(got[3:, good] - ref[3:, good])) <= 1e-12 + tol
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_xy_grids_bit_identical_to_checker(hip, port, seed):
+    """xy grids of every surface type, flipped or not, under random (also
+    scaling) transforms: positions, normals and areas equal the checker's bit
+    for bit (no fused multiply-adds in the generator; the checker is
+    bit-identical to the compiled reference).  An exact 0 matters: the
+    aperture mask of the figures of merit takes atan2(y, x) at the grid
+    point, and at the origin atan2(0, 0) and atan2(1.8e-15, 0) pick different
+    ellipse radii."""
+    rng = np.random.default_rng(seed)
+    n = [int(rng.integers(2, 40)), int(rng.integers(2, 40))]
+    t = seed % 4
+    d = {"gmode": "xy", "gridsize": n, "flip": bool(seed % 3 == 0), "pmode": "manual"}
+    if t == 0:
+        s = G.paraboloid(dict(d, lims_x=[-15, 15], lims_y=[-12, 12], coeffs=list(rng.uniform(20, 80, 2)) + [-1]))
+    elif t == 1:
+        s = G.hyperboloid(dict(d, lims_x=[-10, 10], lims_y=[-10, 10], coeffs=list(rng.uniform(10, 40, 3))))
+    elif t == 2:
+        s = G.ellipsoid(dict(d, lims_x=[-9, 9], lims_y=[-9, 9], coeffs=list(rng.uniform(20, 40, 3))))
+    else:
+        s = G.plane(dict(d, lims_x=[-10, 10], lims_y=[-8, 8]))
+    ax = rng.normal(size=3)
+    m = G.mat_translate(rng.uniform(-20, 20, 3), G.find_rotation(np.array([0, 0, 1.0]), ax / np.linalg.norm(ax)))
+    if seed % 2:
+        m[:3, :3] *= rng.uniform(0.6, 1.8)
+    s["transf"] = m
+    for transform in (True, False):
+        got = B.generate_grid(s, transform, True)
+        ref = OB.generate_grid(port, s, transform=transform)
+        for a in ("x", "y", "z", "nx", "ny", "nz", "area"):
+            np.testing.assert_array_equal(np.ravel(getattr(got, a)), np.ravel(getattr(ref, a)), err_msg=a)
