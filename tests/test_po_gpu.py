@@ -547,3 +547,66 @@ def test_ff_shadow_test_with_scaled_normals(hip, port):
             assert err <= TOL[prec], (kind, prec, err)
     finally:
         B.set_ff_kernel("matrix")
+
+
+@pytest.mark.parametrize("seed", range(60))
+def test_po_random_surfaces_transforms_and_directions(hip, port, seed):
+    """Every mode (by seed) on random surfaces of every type and grid mode,
+    flipped or not, under random rigid and scaling transforms, with random
+    wavelength, exp = fwd or bwd and, in EHP, epsilon; near-field targets are
+    random surfaces 60-200 mm away in a random direction, far-field targets
+    AoE grids up to +-80 deg.  Through the drop-in call, mixed to 1e-5 and
+    FP64 to 1e-10 of each field's peak, same non-finite entries.  A pair
+    whose |n.R| is within rounding of 0 (below 1e-5 for the mixed check,
+    1e-12 for the FP64 one) makes the shadow test a rounding matter, so such
+    cases skip that precision."""
+    from helpers import random_surface
+    rng = np.random.default_rng(5000 + seed)
+    src = random_surface(rng)
+    n = gridsize(src)
+    lam = float(10 ** rng.uniform(np.log10(0.3), np.log10(5.0)))
+    k = 2 * np.pi / lam
+    mode = ("FF", "JM", "EH", "JMEH", "EHP", "scalar")[seed % 6]
+    exp = "fwd" if rng.random() < 0.7 else "bwd"
+    eps = float(rng.uniform(1.0, 4.0)) if mode == "EHP" and rng.random() < 0.5 else 1.0
+    gs = OB.generate_grid(port, src)
+    nrm = np.stack([np.ravel(gs.nx), np.ravel(gs.ny), np.ravel(gs.nz)])
+    pts = np.stack([np.ravel(gs.x), np.ravel(gs.y), np.ravel(gs.z)])
+    if mode == "FF":
+        deg = float(rng.uniform(1, 80))
+        tgt = G.plane({"gmode": "AoE", "lims_Az": [-deg, deg], "lims_El": [-deg, deg],
+                       "gridsize": [int(rng.integers(3, 30)), int(rng.integers(3, 30))]})
+        gt = OB.generate_grid(port, tgt)
+        th, ph = np.ravel(gt.x), np.ravel(gt.y)
+        dots = nrm.T @ np.stack([np.cos(ph) * np.sin(th), np.sin(ph) * np.sin(th), np.cos(th)])
+    else:
+        tgt = random_surface(rng)
+        off = rng.normal(size=3)
+        tgt["transf"] = G.mat_translate(off * rng.uniform(60, 200) / np.linalg.norm(off), tgt["transf"])
+        gt = OB.generate_grid(port, tgt)
+        R = np.stack([np.ravel(gt.x), np.ravel(gt.y), np.ravel(gt.z)])[:, None, :] - pts[:, :, None]
+        dots = np.einsum("is,ist->st", nrm, R / np.linalg.norm(R, axis=0))
+    mind = 1.0 if mode == "scalar" else float(np.min(np.abs(dots)))   # the scalar sum has no shadow test
+    if mode == "scalar":
+        obj, key = T.scalarfield(rng.normal(size=n) + 1j * rng.normal(size=n)), "s_scalarfield"
+    else:
+        comps = [rng.normal(size=n) + 1j * rng.normal(size=n) for _ in range(6)]
+        if rng.random() < 0.3:
+            comps[3:] = [np.zeros(n, complex)] * 3
+        obj, key = T.currents(*comps), "s_current"
+    ref = flat(OB.propagate(port, mode, src, tgt, obj, k, epsilon=eps, exp=exp)[0])
+    checked = 0
+    for prec, floor in (("mixed", 1e-5), ("fp64", 1e-12)):
+        if mind < floor:
+            continue
+        out = flat(B.PyPO_HIPd(src, tgt, {"mode": mode, "k": k, key: obj, "precision": prec, "exp": exp,
+                                          "epsilon": eps}))
+        fin = np.isfinite(ref)
+        np.testing.assert_array_equal(np.isfinite(out), fin)
+        cols = fin.all(0)
+        err = group_err(out[:, cols], ref[:, cols]) if cols.any() else 0.0
+        print(f"seed {seed} {mode} {prec} {exp} eps {eps:.2f}: type {src['type']} gmode {src['gmode']} "
+              f"{n}, min|n.R| {mind:.1e}: {err:.2e}")
+        assert err <= TOL[prec]
+        checked += 1
+    assert checked or mind < 1e-12

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import cases as C
+from helpers import random_surface
 from oracle import bind as OB
 from pypo_amd import bind as B
 from pypo_amd import geometry as G
@@ -160,37 +161,6 @@ def test_ellipsoid_and_plane_full_size_against_cpu(hip, port):
     assert np.max(miss) < 1e-6
 
 
-def _random_surface(rng):
-    """A random quadric or plane (xy or uv grid, maybe flipped) under a random
-    rigid transform, sometimes with a scaling rotation part."""
-    t = int(rng.integers(4))
-    n = [3, 3]
-    flip = bool(rng.random() < 0.3)
-    xy = rng.random() < 0.5
-    if t == 0:
-        a, b = rng.uniform(20, 80, 2)
-        s = G.paraboloid({"gmode": "xy" if xy else "uv", "lims_x": [-15, 15], "lims_y": [-12, 12],
-                          "lims_u": [0, 15], "lims_v": [0, 360], "pmode": "manual", "coeffs": [a, b, -1],
-                          "gridsize": n, "flip": flip})
-    elif t == 1:
-        s = G.hyperboloid({"gmode": "xy" if xy else "uv", "lims_x": [-10, 10], "lims_y": [-10, 10],
-                           "lims_u": [0, 10], "lims_v": [0, 360], "pmode": "manual",
-                           "coeffs": list(rng.uniform(10, 40, 3)), "gridsize": n, "flip": flip})
-    elif t == 2:
-        a, b, c = rng.uniform(20, 40, 3)
-        s = G.ellipsoid({"gmode": "uv", "lims_u": [0, 0.7 * min(a, b)], "lims_v": [0, 360], "pmode": "manual",
-                         "coeffs": [a, b, c], "gridsize": n, "flip": flip})
-    else:
-        s = G.plane({"gmode": "xy" if xy else "uv", "lims_x": [-10, 10], "lims_y": [-8, 8], "lims_u": [0, 10],
-                     "lims_v": [0, 360], "gridsize": n, "flip": flip})
-    ax = rng.normal(size=3)
-    m = G.mat_translate(rng.uniform(-20, 20, 3), G.find_rotation(np.array([0, 0, 1.0]), ax / np.linalg.norm(ax)))
-    if rng.random() < 0.3:
-        m[:3, :3] *= rng.uniform(0.6, 1.8)
-    s["transf"] = m @ s["transf"]
-    return s
-
-
 def _surface_residual(surf, p):
     """|f| of the surface equation at points p (3, n), in the surface's own frame."""
     minv = np.linalg.inv(surf["transf"])
@@ -217,7 +187,7 @@ def test_rt_random_surfaces_and_frames(hip, port, seed):
     rounding differences move arbitrarily; grazing and very distant hits
     agree only to the conditioning of their root.)"""
     rng = np.random.default_rng(9000 + seed)
-    surf = _random_surface(rng)
+    surf = random_surface(rng, (3, 3))
     nr = int(rng.integers(1, 3000))
     o = surf["transf"][:3, 3]
     start = rng.normal(size=(3, nr))
