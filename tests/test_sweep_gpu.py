@@ -163,3 +163,53 @@ def test_full_size_sweep(hip, port):
         assert e_jm <= 1e-5 and e_ff <= 1e-5
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_sweeps_against_checker(hip, port, seed):
+    """Random sweeps: 1-9 instances, feeds of 3-19 points a side at the focus
+    of uv or xy paraboloids of 8-69 points a side (some under a scaling
+    transform), random wavelengths, a shared far field of odd grid sizes
+    (so the main beam is sampled) over a random cone, random launch shapes,
+    graph or eager.  Every instance's JM and FF within 1e-5 of peak of the
+    checker."""
+    from pypo_amd import geometry as G
+    from pypo_amd import types as T
+    rng = np.random.default_rng(7100 + seed)
+    feeds, dishes, ks, curs = [], [], [], []
+    for _ in range(int(rng.integers(1, 10))):
+        nf = [int(rng.integers(3, 20)), int(rng.integers(3, 20))]
+        feed = G.plane({"gmode": "xy", "lims_x": [-4, 4], "lims_y": [-4, 4], "gridsize": nf, "flip": True})
+        f = float(rng.uniform(30, 60))
+        feed["transf"] = G.mat_translate([rng.uniform(-3, 3), rng.uniform(-3, 3), f])
+        nd = [int(rng.integers(8, 70)), int(rng.integers(8, 70))]
+        lims = {"gmode": "uv", "lims_u": [0, 40], "lims_v": [0, 360]} if rng.random() < 0.5 else \
+            {"gmode": "xy", "lims_x": [-30, 30], "lims_y": [-30, 30]}
+        dish = G.paraboloid(dict(lims, pmode="focus", vertex=[0, 0, 0], focus_1=[0, 0, f], gridsize=nd))
+        if rng.random() < 0.3:
+            m = np.eye(4)
+            m[:3, :3] *= rng.uniform(0.7, 1.5)
+            dish["transf"] = m @ dish["transf"]
+        x, y = np.linspace(-4, 4, nf[0])[:, None], np.linspace(-4, 4, nf[1])[None, :]
+        amp = np.exp(-(x ** 2 + y ** 2) / rng.uniform(2, 8)) * np.exp(1j * rng.uniform(0, 0.3) * x)
+        z = np.zeros(nf, complex)
+        feeds.append(feed)
+        dishes.append(dish)
+        ks.append(2 * np.pi / float(rng.uniform(0.8, 4.0)))
+        curs.append(T.currents(-amp + 0.1 * rng.normal(size=nf), z, z, z, amp, z))
+    deg = float(rng.uniform(0.5, 30))
+    ff = G.plane({"gmode": "AoE", "lims_Az": [-deg, deg], "lims_El": [-deg, deg],
+                  "gridsize": [2 * int(rng.integers(1, 20)) + 1, 2 * int(rng.integers(1, 20)) + 1]})
+    shape = dict(tpl=int(rng.choice([1, 2, 4])), chunk_tiles=int(rng.choice([1, 2, 5, 4096])),
+                 wgs_per_cu=int(rng.choice([1, 3, 64, 4096])), use_graph=bool(rng.random() < 0.5))
+    s = SW.Sweep(feeds, dishes, ks, curs, ff, **shape)
+    try:
+        s.run()
+        for i in range(len(feeds)):
+            jm, _, _ = OB.propagate(port, "JM", feeds[i], dishes[i], curs[i], ks[i])
+            e, _, _ = OB.propagate(port, "FF", dishes[i], ff, jm, ks[i])
+            ej, ef = group_err(flat(s.read_currents(i)), flat(jm)), group_err(flat(s.read(i)), flat(e))
+            print(f"seed {seed} instance {i} {shape}: JM {ej:.2e} FF {ef:.2e}")
+            assert ej <= 1e-5 and ef <= 1e-5
+    finally:
+        s.close()
