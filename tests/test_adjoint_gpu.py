@@ -275,3 +275,37 @@ def test_gradient_with_scaled_dish_normals(hip, port):
             assert worst <= bar, (kind, prec, worst)
     finally:
         B.set_ff_kernel("matrix")
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_gradients_against_checker(hip, port, seed):
+    """Random small surface-error problems: Zernike displacement RMS 0.01-0.2
+    mm, far field over +-1..80 deg (wide ones put pairs in the shadow test),
+    four in ten dishes under a scaling transform; the matrix and vector
+    mixed reverse sums to 1e-5 and the FP64 one to 1e-10 of the largest
+    gradient, against the checker."""
+    from pypo_amd import bind as B
+    from helpers import grid_rows
+    rng = np.random.default_rng(3100 + seed)
+    c = small_shape_case(port, seed=seed, sigma=float(rng.uniform(0.01, 0.2)), ff_deg=float(rng.uniform(1, 80)))
+    if rng.random() < 0.4:
+        m = np.eye(4)
+        m[:3, :3] *= rng.uniform(0.7, 1.5)
+        c["dish"]["transf"] = m @ c["dish"]["transf"]
+        c["dr"] = grid_rows(port, c["dish"])
+    prob = PA.ShapeProblem(c["feed"], c["cur"], c["dish"], c["ff"], c["k"])
+    d = torch.tensor(c["d"], dtype=torch.float64, device="cuda")
+    E, _ = OA.forward(port, c["fr"], c["fcells"], c["fc"], c["dr"], c["dcells"], c["d"], c["ffr"], c["k"])
+    g = np.random.default_rng(seed).standard_normal(E.shape[1]) / np.abs(E).max() ** 2
+    ref = OA.power_grad(c["fr"], c["fcells"], c["fc"], OA.displace(c["dr"], c["d"]), c["dcells"], c["ffr"],
+                        E, g, c["k"])
+    try:
+        for kind, prec, bar in (("matrix", "mixed", 1e-5), ("vector", "mixed", 1e-5), ("matrix", "fp64", 1e-10)):
+            B.set_ff_kernel(kind)
+            got = prob.power_grad(d, torch.tensor(E, device="cuda"), torch.tensor(g, device="cuda"),
+                                  precision=prec).cpu().numpy()
+            worst = np.abs(got - ref).max() / np.abs(ref).max()
+            print(f"seed {seed} {kind} {prec}: {worst:.2e}")
+            assert worst <= bar
+    finally:
+        B.set_ff_kernel("matrix")
