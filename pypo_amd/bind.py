@@ -52,8 +52,7 @@ def loadHIPlib():
         except ImportError:
             pass
         if not os.path.exists(LIB_PATH):
-            raise OSError(f"{LIB_PATH} is missing: build it with "
-                          "`python -c 'import __graft_entry__ as g; g.build()'`")
+            raise OSError(f"{LIB_PATH} is missing: build it with `python -m pypo_amd.build_lib`")
         lib = ctypes.CDLL(LIB_PATH)
         P = ctypes.POINTER
         d, i = ctypes.c_double, ctypes.c_int

@@ -1,6 +1,6 @@
 """Build libpypohip.so in-tree with hipcc for gfx950.
 
-Usage: python -m pypo_amd.build_lib   (or __graft_entry__.build()).
+Usage: python -m pypo_amd.build_lib
 """
 
 import os

@@ -178,7 +178,7 @@ def test_ff_matrix_phase_forms(hip, port, case):
     assert group_err(outs["matrix"], outs["vector"]) <= 2e-6
 
 
-def _cfg1():
+def _feed_dish_farfield():
     """Plain PO configuration: vector Gaussian-like feed, lambda = 3 mm, on a
     101 x 101 plane over +-20 mm at the focus of a paraboloid (f = 0.5 m,
     D = 0.3 m, uv 151 x 151); JM on the dish, then FF onto 101 x 101 AoE +-1 deg."""
@@ -198,8 +198,8 @@ def _cfg1():
 
 
 @pytest.fixture(scope="module")
-def cfg1_ref(port):
-    lam, feed, dish, ff, cur = _cfg1()
+def feed_dish_farfield_ref(port):
+    lam, feed, dish, ff, cur = _feed_dish_farfield()
     k = 2 * np.pi / lam
     jm, _, _ = OB.propagate(port, "JM", feed, dish, cur, k)
     eh, _, _ = OB.propagate(port, "FF", dish, ff, jm, k)
@@ -207,22 +207,22 @@ def cfg1_ref(port):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_cfg1_jm_then_ff(hip, cfg1_ref, prec):
-    lam, feed, dish, ff, cur = _cfg1()
+def test_feed_101_dish_151_farfield_101_chain(hip, feed_dish_farfield_ref, prec):
+    lam, feed, dish, ff, cur = _feed_dish_farfield()
     k = 2 * np.pi / lam
-    jm_ref, eh_ref = cfg1_ref
+    jm_ref, eh_ref = feed_dish_farfield_ref
     jm = B.PyPO_HIPd(feed, dish, {"mode": "JM", "k": k, "s_current": cur, "precision": prec})
     e1 = group_err(flat(jm), flat(jm_ref))
     eh = B.PyPO_HIPd(dish, ff, {"mode": "FF", "k": k, "s_current": jm_ref, "precision": prec})
     e2 = group_err(flat(eh), flat(eh_ref))
-    print(f"cfg1 {prec}: JM {e1:.2e} FF {e2:.2e}")
+    print(f"feed 101^2 -> dish 151^2 -> FF 101^2, {prec}: JM {e1:.2e} FF {e2:.2e}")
     assert e1 <= TOL[prec] and e2 <= TOL[prec]
 
 
 def test_linearity_and_device_chaining(hip):
     import torch
     from pypo_amd import device as D
-    lam, feed, dish, ff, cur = _cfg1()
+    lam, feed, dish, ff, cur = _feed_dish_farfield()
     k = 2 * np.pi / lam
     g_feed, g_dish, g_ff = D.generate_grid(feed), D.generate_grid(dish), D.generate_grid(ff)
     c1 = D.currents_tensor(cur)
