@@ -610,3 +610,39 @@ def test_po_random_surfaces_transforms_and_directions(hip, port, seed):
         assert err <= TOL[prec]
         checked += 1
     assert checked or mind < 1e-12
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_many_source_splits_every_mode(hip, port, seed):
+    """3 600-25 600 sources over one to three target blocks: the sums split
+    the sources into eight or more parts, so the workgroups take their work
+    items in the XCD-aware order (each XCD a contiguous run of the
+    split-major list), with a partial last tile.  Every mode (by seed),
+    mixed to 1e-5 and FP64 to 1e-10 of peak against the checker."""
+    rng = np.random.default_rng(27000 + seed)
+    mode = ("FF", "JM", "EH", "JMEH", "EHP", "scalar")[seed % 6]
+    n = [int(rng.integers(60, 160)), int(rng.integers(60, 160))]
+    src = G.paraboloid({"pmode": "focus", "vertex": [0, 0, 0], "focus_1": [0, 0, 80.0], "gmode": "uv",
+                        "lims_u": [0, 60], "lims_v": [0, 360], "gridsize": n})
+    k = 2 * np.pi / float(rng.uniform(1.0, 4.0))
+    if mode == "FF":
+        deg = float(rng.uniform(1, 20))
+        tgt = G.plane({"gmode": "AoE", "lims_Az": [-deg, deg], "lims_El": [-deg, deg],
+                       "gridsize": [int(rng.integers(3, 30)), int(rng.integers(3, 30))]})
+    else:
+        tgt = G.plane({"gmode": "xy", "lims_x": [-10, 10], "lims_y": [-10, 10],
+                       "gridsize": [int(rng.integers(3, 25)), int(rng.integers(3, 25))], "flip": True})
+        tgt["transf"] = G.mat_translate([0, 0, 80.0])
+    if mode == "scalar":
+        obj, key = T.scalarfield(rng.normal(size=n) + 1j * rng.normal(size=n)), "s_scalarfield"
+    else:
+        x = np.linspace(-1, 1, n[0])[:, None]
+        amp = np.exp(-x ** 2) * np.ones(n) + 0.1 * rng.normal(size=n)
+        z = np.zeros(n, complex)
+        obj, key = T.currents(amp + 0j, 0.3 * amp + 0j, z, z, z, z), "s_current"
+    ref = flat(OB.propagate(port, mode, src, tgt, obj, k)[0])
+    for prec in ("mixed", "fp64"):
+        out = flat(B.PyPO_HIPd(src, tgt, {"mode": mode, "k": k, key: obj, "precision": prec}))
+        err = group_err(out, ref)
+        print(f"seed {seed} {mode} {prec}: {n[0]}x{n[1]} sources: {err:.2e}")
+        assert err <= TOL[prec]
