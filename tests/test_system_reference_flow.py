@@ -134,3 +134,26 @@ def test_interp_linear_and_cubic(s):
     with pytest.raises(ValueError):
         s.interpFrame("cloud", "cloud_f", PLANE_XY["name"], "cub_out", method="cubic")
     assert fr.size == 169
+
+
+def test_mergeBeams(s):
+    """test_SystemPO_RT.py:198-202: merging the Gaussian and the point source on
+    the plane stores a field and a current under the merged name (here also:
+    the merge is their sum)."""
+    s.mergeBeams(GPO_FIELD["name"], PS_UFIELD["name"], merged_name="test")
+    s.mergeBeams(GPO_FIELD["name"], PS_UFIELD["name"], obj="currents", merged_name="test")
+    assert "test" in s.fields and "test" in s.currents
+    assert type(s.fields["test"]) is T.fields and type(s.currents["test"]) is T.currents
+    want = flat(s.fields[GPO_FIELD["name"]]) + flat(s.fields[PS_UFIELD["name"]])
+    np.testing.assert_allclose(flat(s.fields["test"]), want, rtol=0, atol=1e-15 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("name", [GPO_FIELD["name"], PS_UFIELD["name"]])
+def test_interpBeam(s, name):
+    """test_SystemPO_RT.py:204-212: interpolating a source's field and currents
+    onto a 101 x 101 copy of its plane stores both under name + "_interp"."""
+    s.interpBeam(name, np.array([101, 101]), obj="fields")
+    s.interpBeam(name, np.array([101, 101]), obj="currents")
+    assert f"{name}_interp" in s.fields and f"{name}_interp" in s.currents
+    assert s.fields[f"{name}_interp"].shape == (101, 101)
+    assert s.system[PLANE_XY["name"] + "_interp"]["gridsize"].tolist() == [101, 101]
