@@ -1,9 +1,10 @@
-"""The reference's own System tests (tests/test_SystemPO_RT.py:47-152 with
+"""The reference's own System tests (tests/test_SystemPO_RT.py:47-212 with
 the templates of tests/TestTemplates.py) run through pypo_amd.system.System
 with the reference's default device="CPU": JM, EH and JMEH fwd and bwd onto
 the manual xy paraboloid, FF onto the AoE plane, EHP followed by the hybrid
 step (interpolated with a start point, and without interpolation for one
-component), scalar mode and a ray trace of the tubular frame.  As there,
+component), scalar mode, a ray trace of the tubular frame, autoConverge onto
+the manual hyperboloids, mergeBeams and interpBeam.  As there,
 the result types are checked; here the values are also compared with the
 CPU checker wherever the checker's value is finite (the templates put a
 source and a target point at the origin, where both give NaN)."""
@@ -157,3 +158,25 @@ def test_interpBeam(s, name):
     assert f"{name}_interp" in s.fields and f"{name}_interp" in s.currents
     assert s.fields[f"{name}_interp"].shape == (101, 101)
     assert s.system[PLANE_XY["name"] + "_interp"]["gridsize"].tolist() == [101, 101]
+
+
+HYP_MAN_XY = {"name": "testHyperboloid_man_xy", "pmode": "manual", "coeffs": np.array([1, 1, 1]), "ecc": 1.1,
+              "gmode": "xy", "lims_x": np.array([-1, 1]), "lims_y": np.array([-1, 1]),
+              "gridsize": np.array([13, 13])}
+HYP_MAN_UV = {"name": "testHyperboloid_man_uv", "pmode": "manual", "coeffs": np.array([1, 1, 1]), "ecc": 1.1,
+              "gmode": "uv", "lims_u": np.array([0, 1]), "lims_v": np.array([0, 360]),
+              "gridsize": np.array([13, 12])}
+
+
+@pytest.mark.parametrize("name", [GPO_FIELD["name"], PS_UFIELD["name"]])
+def test_autoConverge(s, name):
+    """test_SystemPO_RT.py:187-196: autoConverge from each source onto the
+    manual xy and uv hyperboloids returns a gridsize array and sets it on
+    the surface."""
+    s.addHyperbola(dict(HYP_MAN_XY))
+    s.addHyperbola(dict(HYP_MAN_UV))
+    for hyp in (HYP_MAN_XY, HYP_MAN_UV):
+        gridsize = s.autoConverge(name, hyp["name"])
+        assert type(gridsize) is np.ndarray
+        assert gridsize[0] == s.system[hyp["name"]]["gridsize"][0]
+        assert gridsize[1] == s.system[hyp["name"]]["gridsize"][1]
